@@ -1,0 +1,146 @@
+"""Benchmark of the MBIR_4D ICD/NHICD hot path on one MI355X.
+
+Workload (one GPU): 4D interlaced-view MBIR, finest multi-resolution stage of
+a 256 x 256 x 8-slice Cahn-Hilliard microstructure over 16 time frames, 1024
+interlaced views (K = 8 sub-frames) over one half turn, 256 detector bins,
+Poisson I0 = 1e4, zingers in 0.1% of the bins, per-detector ring offsets
+N(0, 0.01); generalized-Huber data term, ring correction and variance
+estimation on.  A step is one full ICD iteration of that stage (HICD + NHICD
+colour phases, offset solves after every phase, VSC selection, variance
+update, convergence check), inputs resident in HBM.
+
+Prints one JSON line.  ``value`` is voxel updates per second over the timed
+steps (wall clock, all kernels included); the roofline block covers the
+voxel-update kernel alone (HIP events on the library's stream).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "time-to-converged MAP (1e-4 rel) & voxel updates/s, learned vs plain init"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_scan(small: bool):
+    from timbir_amd import datagen
+    if small:
+        return datagen.interlaced4d(N=64, nz=4, n_frames=4, n_views=256, K=8, ch_steps=16,
+                                    frame_steps=2)
+    return datagen.interlaced4d()
+
+
+def cpu_baseline(scan, stage_kw, threads: int, budget_s: float = 20.0):
+    """The oracle (CPU restatement, reference schedule) on the same stage:
+    voxel updates per second over whole ICD iterations, bounded in time."""
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import pyorc
+    o = pyorc.Problem(scan, opt=pyorc.options(schedule=pyorc.SCHED_REFERENCE, num_threads=threads),
+                      **stage_kw)
+    o.init_error_sino()
+    t0 = time.perf_counter()
+    it = 0
+    while True:
+        it += 1
+        o.icd_iteration(it)
+        el = time.perf_counter() - t0
+        if el > budget_s or it >= 3:
+            break
+    upd = o.update_count
+    return dict(value=upd / el, unit="voxel updates/s", cores=threads, kind="port",
+                sample=f"{it} full ICD iteration(s) of the same stage, reference schedule, "
+                       f"{upd} voxel updates in {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--small", action="store_true", help="tiny workload for a quick check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
+                                   "(the z-slab multi-GPU split is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    import timbir_amd as tb
+    tb.set_options(verbose=0, write_files=0)
+    t0 = time.perf_counter()
+    scan = make_scan(args.small)
+    log(f"scan generated in {time.perf_counter() - t0:.1f} s")
+    # the finest stage of reconstruct()'s schedule for this scan: (1, 1) after
+    # (4, 4) and (2, 2)
+    stage_kw = dict(mult_xy=1, mult_z=1, stage_idx=2, prev_mult_z=2)
+    st = tb.Stage(scan, **stage_kw)
+    info = st.info
+    st.init_error_sino()
+    it = 0
+    for _ in range(args.warmup):
+        it += 1
+        st.icd_iteration(it)
+    st.synchronize()
+    st.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it += 1
+        st.icd_iteration(it)
+    st.synchronize()
+    el = time.perf_counter() - t0
+    s = st.stats()
+    cost = st.cost()[0]
+    value = s["updates"] / el
+    vox_s = s["voxel_ms"] / 1e3
+    achieved = s["alg_bytes"] / vox_s / 1e9 if vox_s > 0 else 0.0
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Cahn-Hilliard phantom, Poisson counts, zingers, rings)",
+        "config": {
+            "workload": "4D interlaced-view MBIR, finest stage: 256x256x8 slices x 16 frames, "
+                        "1024 views K=8, 256 bins, I0=1e4, zingers 0.1%, rings N(0,0.01)"
+                        if not args.small else "small check workload",
+            "N_x": info["N_x"], "N_z": info["N_z"], "N_time": info["N_time"],
+            "N_p": info["N_p"], "N_r": info["N_r"], "views_per_frame":
+                info["N_p"] // info["N_time"],
+        },
+        "voxel_kernel": {"updates": s["updates"], "zero_skips": s["zero_skips"],
+                         "launches": s["voxel_launches"], "time_ms": s["voxel_ms"],
+                         "updates_per_s": s["updates"] / vox_s if vox_s > 0 else None,
+                         "entries": s["entries"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cost_after": cost,
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)
+    st.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * timbir_dev.h -- C ABI of the GPU-resident MBIR_4D hot path (libtimbir_hip).
+ *
+ * One tb_stage is one multi-resolution stage of reconstruct() held in the
+ * GPU's memory: error sinogram, weights, offsets, object, magnitude-update
+ * map and the detector response table.  Each entry point below replaces one
+ * reference function (cited as file:line under adityamnk/timbir/src/MBIR_4D);
+ * reconstruct() in mbir4d.h is built from them.  Every function returns 0 on
+ * success and -1 on failure (tb_last_error() then describes it), the
+ * reference's check_error convention (XT_Debug.h:22).
+ *
+ * Arrays crossing this boundary are host pointers in the reference's own
+ * layouts:
+ *   raw projections / weights : float [proj_num][proj_rows][proj_cols]
+ *   object                    : float [N_time][N_z][N_y][N_x]
+ *   sinogram-shaped outputs   : float [N_p][N_r][N_t]  (XT_Init.c:298-305)
+ *   offsets                   : float [N_r][N_t]
+ *   magnitude-update map      : float [N_time][N_z][N_y][N_x] (one z-block
+ *                               per slice)
+ * No torch or HIP type appears here.
+ */
+#ifndef TIMBIR_DEV_H_
+#define TIMBIR_DEV_H_
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tb_stage tb_stage;
+
+/* Options the reference fixes with compile-time macros (XT_Constants.h,
+ * src/MBIR_4D/Makefile:37).  Defaults: zeros except max_iterations = 1000. */
+typedef struct {
+  int32_t compute_cost;           /* 1 = build without -DNO_COST_CALCULATE */
+  int32_t abort_on_cost_increase; /* XT_ICD_update.c:1185 */
+  int32_t max_iterations;         /* MAX_NUM_ITERATIONS, XT_Constants.h:86 */
+  int32_t no_nhicd;               /* NO_NHICD, XT_Constants.h:91 */
+  int32_t write_files;            /* write the .bin outputs (default 1) */
+  uint32_t order_seed;            /* seed of the per-cell voxel order */
+  int32_t verbose;                /* [INFO] lines on the debug stream */
+} tb_options;
+
+void tb_get_options(tb_options *opt);
+void tb_set_options(const tb_options *opt);
+const char *tb_last_error(void);
+
+/* initStatStructures + initDynStructures (XT_Init.c:211-669),
+ * DetectorResponseProfile (XT_Profile.c:129-176) and the object mask
+ * (XT_ICD_update.c:1140-1149) for one stage.  Uploads the raw scan; weights
+ * are divided by var_est.  stage_idx counts from the coarsest stage (0);
+ * prev_mult_z is the previous stage's z multiplier. */
+int tb_stage_create(tb_stage **out, const float *projections,
+                    const float *weights, const float *proj_angles,
+                    const float *proj_times, const float *recon_times,
+                    int32_t proj_rows, int32_t proj_cols, int32_t proj_num,
+                    int32_t recon_num, float vox_wid, float rot_center,
+                    float sig_s, float sig_t, float c_s, float c_t,
+                    float convg_thresh, int32_t remove_rings,
+                    int32_t quad_convex, float huber_delta, float huber_T,
+                    int32_t mult_xy, int32_t mult_z, int32_t stage_idx,
+                    int32_t prev_mult_z, double var_est);
+void tb_stage_destroy(tb_stage *st);
+
+/* ints: N_p N_r N_t N_x N_y N_z N_time mult_xy mult_z num_z_blocks
+ * updateProjOffset updateVar initICD U ; dbls as the oracle's
+ * orc_problem_info (delta_r .. delta_recon, then 27 filter weights). */
+int tb_stage_info(const tb_stage *st, int32_t *ints, double *dbls);
+
+/* Object / offset / map transfer: initObject (XT_ICD_update.c:541-651) and
+ * write_ObjectProjOff2TiffBinPerIter (XT_IOMisc.c:238-271). */
+int tb_set_object(tb_stage *st, const float *obj);
+int tb_get_object(tb_stage *st, float *obj);
+int tb_set_offset(tb_stage *st, const float *d);
+int tb_get_offset(tb_stage *st, float *d);
+int tb_set_magmap(tb_stage *st, const float *mag);
+int tb_get_magmap(tb_stage *st, float *mag);
+/* initObject initICD = 2 / 3 from the previous stage's object and map, on
+ * the GPU: bilinear x2 in x-y (and z when mult_z halves). */
+int tb_init_from_coarse(tb_stage *st, const float *prev_obj, int32_t prev_nz,
+                        const float *prev_mag);
+int tb_get_error_sino(tb_stage *st, float *e);  /* [N_p][N_r][N_t] */
+int tb_get_weights(tb_stage *st, float *w);     /* [N_p][N_r][N_t] */
+int tb_get_detector_response(tb_stage *st, float *H_r); /* [N_p][65] */
+double tb_get_var_est(const tb_stage *st);
+
+/* initErrorSinogam (XT_ICD_update.c:655-716): e = y - A x - d with the
+ * forward projection of forward_project_voxel_AMat1D (XT_ForwardProject.c:8-25). */
+int tb_init_error_sino(tb_stage *st);
+/* Forward projection alone (no y, no d): out = A x, [N_p][N_r][N_t]. */
+int tb_forward_project(tb_stage *st, float *out);
+
+/* computeCost (XT_ICD_update.c:186-319), FP64 reductions. */
+int tb_cost(tb_stage *st, double *total, double *forward, double *prior);
+
+/* update_Sinogram_Offset (XT_ICD_update.c:919-927) with the given type
+ * (1 unconstrained :870, 2 zero-mean :817, 3 patch-constrained with no
+ * patches :753). */
+int tb_offset_update(tb_stage *st, int32_t type);
+/* update_variance_parameter (XT_ICD_update.c:720-751). */
+int tb_variance_update(tb_stage *st);
+
+/* compute_voxel_update_Atten (XT_VoxUpdate.c:43-115) for a single voxel,
+ * gathering its neighbourhood as updateVoxels_Atten does (:166-233).
+ * Returns the new value in *value. */
+int tb_voxel_update(tb_stage *st, int32_t frame, int32_t slice, int32_t y,
+                    int32_t x, float *value);
+/* updateVoxels (XT_VoxUpdate.c:284-289) for every cell of one colour:
+ * HICD chunk `chunk` (0..9) of the iteration's voxel order, or the NHICD
+ * lines when nhicd != 0.  Cells (frame f, slice z) of colour (f + z) & 1. */
+int tb_update_colour(tb_stage *st, int32_t colour, int32_t iter,
+                     int32_t chunk, int32_t nhicd);
+/* VSC_based_Voxel_Line_Select (XT_NHICD.c:164-216). */
+int tb_vsc_select(tb_stage *st);
+/* NHICD list of cell (frame, slice), raster order; returns the count. */
+int tb_nhicd_list(tb_stage *st, int32_t frame, int32_t slice, int32_t *xs,
+                  int32_t *ys);
+/* updateVoxelsTimeSlices (XT_ICD_update.c:936-1105): one ICD iteration.
+ * *converged = the reference's convergence flag, *ratio its percentage. */
+int tb_icd_iteration(tb_stage *st, int32_t iter, int32_t *converged,
+                     double *ratio);
+/* ICD_BackProject's loop (XT_ICD_update.c:1108-1244) after the object is
+ * set: error sinogram, iterations until the reference's rule (or
+ * max_iter), cost per iteration when enabled.  costs (max_iter+1 entries)
+ * may be NULL. */
+int tb_stage_run(tb_stage *st, int32_t max_iter, double *costs,
+                 int32_t *iters);
+
+/* Accounting of the GPU work since the last reset, measured with HIP
+ * events on the stage's stream: the voxel-update kernel time (ms), the
+ * voxel updates performed (zero-skipped voxels excluded), and the
+ * algorithmic bytes they moved (12 B per touched A-matrix entry: e read,
+ * w read, e write).  Other kernels' time in other_ms. */
+typedef struct {
+  double voxel_ms;
+  double other_ms;
+  int64_t voxel_launches;
+  int64_t updates;
+  int64_t zero_skips;
+  int64_t entries;
+  double alg_bytes;
+} tb_stats;
+int tb_get_stats(tb_stage *st, tb_stats *out);
+int tb_reset_stats(tb_stage *st);
+int tb_synchronize(tb_stage *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

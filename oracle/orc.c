@@ -1112,6 +1112,10 @@ static double run_phase(orc_problem *p, int32_t phase, int32_t iter, int32_t j, 
   return tot;
 }
 
+double orc_run_phase(orc_problem *p, int32_t phase, int32_t iter, int32_t chunk, int32_t hicd) {
+  return run_phase(p, phase, iter, chunk, hicd);
+}
+
 int32_t orc_icd_iteration(orc_problem *p, int32_t iter, double *ratio) {
   const int K = ORC_NHICD_K;
   size_t nxy = (size_t)p->N_x * p->N_y;

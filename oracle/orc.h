@@ -145,6 +145,11 @@ double orc_update_voxels(orc_problem *p, int32_t frame, int32_t z_begin,
 /* XT_ICD_update.c:936-1105: one ICD iteration.  Returns 1 when the
  * reference's convergence rule fires; *ratio gets the percentage. */
 int32_t orc_icd_iteration(orc_problem *p, int32_t iter, double *ratio);
+/* One colour phase of updateVoxelsTimeSlices (XT_ICD_update.c:977-1053):
+ * phase 0/1, HICD chunk `chunk` (hicd=1) or the NHICD lists (hicd=0).
+ * Returns the summed |x| of the updated voxels.  For the colour schedule
+ * the per-iteration selection counts must be set (orc_set_select_counts). */
+double orc_run_phase(orc_problem *p, int32_t phase, int32_t iter, int32_t chunk, int32_t hicd);
 /* XT_NHICD.c:164-216 for every (frame, block) */
 void orc_vsc_select(orc_problem *p);
 int32_t orc_nhicd_count(const orc_problem *p, int32_t frame, int32_t block,

@@ -120,6 +120,8 @@ def lib(precision: str = "f32") -> C.CDLL:
                                     _i32p, _i32p, C.c_int32, C.c_int32]
     L.orc_icd_iteration.restype = C.c_int32
     L.orc_icd_iteration.argtypes = [vp, C.c_int32, C.POINTER(C.c_double)]
+    L.orc_run_phase.restype = C.c_double
+    L.orc_run_phase.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
     L.orc_vsc_select.argtypes = [vp]
     L.orc_nhicd_count.restype = C.c_int32
     L.orc_nhicd_count.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
@@ -272,6 +274,10 @@ class Problem:
         r = C.c_double()
         flag = self.L.orc_icd_iteration(self.h, it, C.byref(r))
         return flag, r.value
+
+    def run_phase(self, phase, it, chunk, hicd=1):
+        self.L.orc_set_select_counts(self.h, self.info["U"])
+        return self.L.orc_run_phase(self.h, phase, it, chunk, hicd)
 
     def vsc_select(self):
         self.L.orc_vsc_select(self.h)

@@ -1,0 +1,256 @@
+"""Parity of the HIP path (libtimbir_hip.so, called through its C ABI) with the
+CPU oracle on the same seeded inputs.  Integer/index work and the system model
+must match bit for bit; floating-point results within the tolerances stated
+in each test."""
+import sys
+
+import numpy as np
+import pytest
+
+import timbir_amd as tb
+from oracle import pyorc
+from timbir_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def scan_small(**kw):
+    base = dict(N=16, nz=4, n_frames=2, n_views=24, sig_s=0.005, sig_t=0.01, convg_thresh=0.0)
+    base.update(kw)
+    return datagen.tiny(**base)
+
+
+def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, precision="f32"):
+    o = pyorc.Problem(scan, mult_xy=mult_xy, mult_z=mult_z, stage_idx=stage_idx,
+                      prev_mult_z=prev_mult_z, var_est=var_est,
+                      opt=pyorc.options(schedule=pyorc.SCHED_COLOUR), precision=precision)
+    g = tb.Stage(scan, mult_xy=mult_xy, mult_z=mult_z, stage_idx=stage_idx,
+                 prev_mult_z=prev_mult_z, var_est=var_est)
+    return o, g
+
+
+def random_object(shape, mask, seed=1, scale=0.02):
+    rng = np.random.default_rng(seed)
+    return (rng.random(shape) * scale * mask[None, None]).astype(np.float32)
+
+
+@pytest.mark.parametrize("mxy,mz", [(1, 1), (2, 2), (4, 2)])
+def test_stage_setup_matches_oracle(mxy, mz):
+    log(f"setup mxy={mxy} mz={mz}")
+    scan = scan_small(N=32, nz=4)
+    o, g = pair(scan, stage_idx=1 if mxy > 1 else 0, mult_xy=mxy, mult_z=mz,
+                prev_mult_z=2 * mz if mxy > 1 else 0)
+    for k, v in o.info.items():
+        if k == "num_z_blocks":
+            continue
+        assert g.info[k] == v, k
+    np.testing.assert_array_equal(g.sfilt, o.sfilt)
+    # system model and the transposed, variance-scaled weights: bit for bit
+    np.testing.assert_array_equal(g.detector_response(), o.hr())
+    np.testing.assert_array_equal(g.weights(), o.wgt())
+
+
+def test_forward_projection_bitwise():
+    log("forward projection")
+    scan = scan_small(N=32, nz=4, n_views=40)
+    o, g = pair(scan)
+    x = random_object(o.obj_shape, o.mask())
+    o.obj()[...] = x
+    g.set_object(x)
+    ref = o.forward_project()
+    got = g.forward_project()
+    # same A entries, same raster accumulation order, same float rounding
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("mxy,mz", [(1, 1), (2, 2)])
+def test_error_sinogram_init(mxy, mz):
+    log(f"error sino mxy={mxy}")
+    scan = scan_small(N=32, nz=4, n_views=40, rings=0.01, remove_rings=1)
+    o, g = pair(scan, mult_xy=mxy, mult_z=mz)
+    x = random_object(o.obj_shape, o.mask())
+    o.obj()[...] = x
+    g.set_object(x)
+    d = (np.random.default_rng(5).normal(0, 0.01, (o.info["N_r"], o.info["N_t"]))).astype(np.float32)
+    o.off()[...] = d
+    g.set_offset(d)
+    o.init_error_sino()
+    g.init_error_sino()
+    np.testing.assert_array_equal(g.error_sino(), o.err())
+
+
+@pytest.mark.parametrize("quad", [1, 0])
+def test_cost_matches_oracle(quad):
+    log(f"cost quad={quad}")
+    scan = scan_small(N=32, nz=4, n_views=40, quad_convex=quad, zingers=1e-2)
+    o, g = pair(scan)
+    x = random_object(o.obj_shape, o.mask())
+    o.obj()[...] = x
+    g.set_object(x)
+    o.init_error_sino()
+    g.init_error_sino()
+    co = o.cost()
+    cg = g.cost()
+    # FP64 reductions of identical float inputs in different orders
+    for a, b in zip(cg, co):
+        assert abs(a - b) <= 1e-11 * abs(b) + 1e-12
+
+
+def test_single_voxel_update():
+    log("single voxel")
+    scan = scan_small(N=32, nz=4, n_views=40)
+    o, g = pair(scan)
+    x = random_object(o.obj_shape, o.mask(), seed=3)
+    o.obj()[...] = x
+    g.set_object(x)
+    o.init_error_sino()
+    g.init_error_sino()
+    for (f, z, yy, xx) in [(0, 1, 16, 16), (1, 2, 10, 20), (0, 0, 5, 12), (1, 3, 27, 16)]:
+        vo = o.voxel_update(f, z, yy, xx)
+        vg = g.voxel_update(f, z, yy, xx)
+        # theta sums: FP32 compensated (GPU) vs FP64 (oracle)
+        assert abs(vg - vo) <= 2e-5 * max(abs(vo), 1e-3), (f, z, yy, xx, vg, vo)
+    e_o = o.err()
+    e_g = g.error_sino()
+    assert np.max(np.abs(e_g - e_o)) <= 2e-5 * max(1.0, np.abs(e_o).max())
+
+
+@pytest.mark.parametrize("typ", [1, 2, 3])
+def test_offset_update(typ):
+    log(f"offset type {typ}")
+    scan = scan_small(N=32, nz=4, n_views=40, rings=0.02, quad_convex=0, zingers=1e-2,
+                      remove_rings=typ)
+    o, g = pair(scan)
+    x = random_object(o.obj_shape, o.mask(), seed=4)
+    o.obj()[...] = x
+    g.set_object(x)
+    o.init_error_sino()
+    g.init_error_sino()
+    o.offset_update(typ)
+    g.offset_update(typ)
+    do, dg = o.off(), g.offset()
+    np.testing.assert_allclose(dg, do, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(g.error_sino(), o.err(), rtol=1e-5, atol=1e-6)
+    if typ == 2:
+        assert abs(float(dg.astype(np.float64).sum())) < 1e-4
+
+
+def test_variance_update():
+    log("variance")
+    scan = scan_small(N=32, nz=4, n_views=40, quad_convex=0, zingers=1e-2)
+    o, g = pair(scan, stage_idx=1, mult_xy=2, mult_z=2, prev_mult_z=4)
+    x = random_object(o.obj_shape, o.mask(), seed=6)
+    o.obj()[...] = x
+    g.set_object(x)
+    o.init_error_sino()
+    g.init_error_sino()
+    o.variance_update()
+    g.variance_update()
+    assert abs(g.var_est - o.var_est) <= 1e-12 * abs(o.var_est)
+    np.testing.assert_array_equal(g.weights(), o.wgt())
+
+
+def test_vsc_select_lists():
+    log("vsc select")
+    scan = scan_small(N=32, nz=4, n_views=40)
+    o, g = pair(scan)
+    rng = np.random.default_rng(9)
+    mag = rng.random(o.obj_shape).astype(np.float32)
+    mag[0, 1, 3:9, :] = 0.0
+    o.mag()[...] = mag
+    g.set_magmap(mag)
+    o.L.orc_set_select_counts(o.h, o.info["U"])
+    o.vsc_select()
+    g.vsc_select()
+    for f in range(o.info["N_time"]):
+        for z in range(o.info["N_z"]):
+            xo, yo = o.nhicd_list(f, z)
+            xg, yg = g.nhicd_list(f, z)
+            np.testing.assert_array_equal(xg, xo)
+            np.testing.assert_array_equal(yg, yo)
+
+
+@pytest.mark.parametrize("quad,rings", [(1, 0), (0, 2), (0, 1)])
+def test_icd_iterations_track_oracle(quad, rings):
+    """Same colour schedule on both sides: the trajectories agree to FP32
+    (non-stiff prior, see test_phases_track_oracle)."""
+    log(f"icd trajectory quad={quad} rings={rings}")
+    scan = scan_small(N=32, nz=4, n_views=40, quad_convex=quad, rings=0.01 if rings else 0.0,
+                      zingers=1e-3 if not quad else 0.0, remove_rings=rings, c_s=1.0, c_t=1.0)
+    o, g = pair(scan, stage_idx=1 if rings else 0, prev_mult_z=1 if rings else 0)
+    o.init_error_sino()
+    g.init_error_sino()
+    for it in range(1, 5):
+        fo, ro = o.icd_iteration(it)
+        fg, rg = g.icd_iteration(it)
+        xo, xg = o.obj(), g.object()
+        rng_ = xo.max() - xo.min()
+        rmse = np.sqrt(np.mean((xg - xo) ** 2)) / rng_
+        co, cg = o.cost()[0], g.cost()[0]
+        log(f"  it {it}: rmse/range {rmse:.2e} cost {cg:.8e} vs {co:.8e} ratio {rg:.4f} vs {ro:.4f}")
+        assert rmse < 1e-4
+        assert abs(cg - co) <= 1e-5 * abs(co)
+
+
+def test_converges_to_map():
+    """GPU ICD reaches the scipy minimiser of the explicit MAP cost."""
+    import mathref
+    log("converge to MAP")
+    scan = scan_small(N=12, nz=4, n_views=16)
+    o = pyorc.Problem(scan, opt=pyorc.options(), precision="f64")
+    mp = mathref.MapProblem(o)
+    cmin, xmin = mp.minimise()
+    g = tb.Stage(scan)
+    costs = g.stage_run(300)
+    x = g.object()
+    rmse = np.sqrt(np.mean((x - xmin) ** 2)) / (xmin.max() - xmin.min())
+    log(f"  gpu {costs[-1]:.10e} scipy {cmin:.10e} rmse {rmse:.2e}")
+    assert abs(costs[-1] - cmin) <= 1e-5 * abs(cmin)
+    assert rmse <= 5e-3
+
+
+def test_phases_track_oracle(c=1.0):
+    """Colour phases of iterations 1-2 one at a time (HICD, VSC, NHICD).
+
+    With a stiff prior (c ~ 1e-2, sigma ~ 5e-3) early ICD trajectories are
+    chaotic: two CPU oracles that differ only in float vs double arrays part
+    by ~1% RMSE at iteration 2.  Trajectory parity is therefore checked with
+    c = 1; stiff problems are checked at convergence (test_converges_to_map)."""
+    log(f"phase by phase c={c}")
+    scan = scan_small(N=32, nz=4, n_views=40, c_s=c, c_t=c)
+    o, g = pair(scan)
+    o.init_error_sino()
+    g.init_error_sino()
+    o.L.orc_set_select_counts(o.h, o.info["U"])
+    for it in (1, 2):
+        for j in range(10):
+            steps = [("hicd0", 0, 1), ("hicd1", 1, 1), ("vsc", None, None)]
+            if it > 1:
+                steps += [("nhicd0", 0, 0), ("nhicd1", 1, 0)]
+            for name, col, hicd in steps:
+                if name == "vsc":
+                    o.vsc_select()
+                    g.vsc_select()
+                    for f in range(o.info["N_time"]):
+                        for z in range(o.info["N_z"]):
+                            xo, yo = o.nhicd_list(f, z)
+                            xg, yg = g.nhicd_list(f, z)
+                            same = len(xo) == len(xg) and np.array_equal(xo, xg) and np.array_equal(yo, yg)
+                            assert same, (it, j, f, z, len(xo), len(xg))
+                    continue
+                o.run_phase(col, it, j, hicd)
+                g.update_colour(col, it, j, 1 - hicd)
+                xo, xg = o.obj(), g.object()
+                err = np.max(np.abs(xg - xo)) / (np.abs(xo).max() + 1e-30)
+                if err > 1e-4:
+                    d = np.argwhere(np.abs(xg - xo) > 1e-4 * np.abs(xo).max())
+                    log(f"diverged at it {it} chunk {j} {name}: {err:.2e}, {len(d)} voxels, first {d[:5].tolist()}")
+                    log(f"  values gpu {[float(xg[tuple(q)]) for q in d[:5]]} oracle {[float(xo[tuple(q)]) for q in d[:5]]}")
+                    mg, mo = g.magmap(), o.mag()
+                    log(f"  mag diff max {np.max(np.abs(mg - mo)):.3e}")
+                assert err <= 1e-4, (it, j, name, err)

@@ -1,0 +1,98 @@
+// Shared host/device definitions of libtimbir_hip.
+#pragma once
+#include <stdint.h>
+#include <stddef.h>
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#endif
+
+namespace tb {
+
+// XT_Constants.h:62-102 of the reference
+constexpr double kMrfP = 1.2;
+constexpr double kMrfQ = 2.0;
+constexpr int kProfileRes = 1536;
+constexpr int kBeamRes = 512;
+constexpr int kRespBins = 64;  // DETECTOR_RESPONSE_BINS; rows hold 65 values
+constexpr int kHrStride = kRespBins + 1;
+constexpr double kAlpha = 1.5;
+constexpr int kMaxIters = 1000;
+constexpr double kZingerOffT = 100000.0;
+constexpr double kZingerOffDelta = 1.0;
+constexpr int kMinXYRes = 64;
+constexpr int kMaxMultres = 8;
+constexpr int kMinRowsPerNode = 2;
+constexpr int kMinProjRows = 4;
+constexpr int kNhicdK = 10;  // NHICD_Iterations, XT_Init.c:645
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kPi2 = 1.57079632679489661923132169163975144;
+constexpr double kPi4 = 0.785398163397448309615660845819875721;
+
+// Everything a kernel needs, passed by value.  Device sinogram layout is
+// [N_z][N_p][N_r][mz] (detector row t = z*mz + m): a cell (frame, slice)
+// reads one contiguous [views][N_r][mz] block.
+struct KP {
+  int N_p, N_r, N_t, N_x, N_y, N_z, N_time, mz, K;
+  double R0, RMax, delta_r, x0, y0, delta_xy, OffsetR;
+  float sSq, sSqp, sTq, sTqp, c_s, c_t, tfilt, alpha;
+  float sfilt[27];
+  float T, Td;       // generalized Huber threshold and T*delta
+  double T_d, Td_d;  // same in double
+  double dl_d;       // generalized Huber delta
+  float *e, *w, *y, *d, *obj, *mag;
+  const float *hr;
+  const double *cosv, *sinv;
+  const int *view_off, *view_idx;
+  const uint8_t *mask;
+  int *nh_list, *nh_count;
+  int nh_cap;
+  double *cell_voxmag;
+  unsigned long long *cell_upd, *cell_zero, *cell_ent;
+  uint32_t order_seed;
+};
+
+// Colour-schedule voxel order: keyed 4-round Feistel bijection on
+// [0, 2^(2h)) with cycle walking onto [0, N).  No reference counterpart
+// (the reference shuffles with random2, XT_ICD_update.c:500-532).
+#if defined(__HIPCC__) || defined(__HIP__)
+#define TB_HD __host__ __device__ __forceinline__
+#else
+#define TB_HD inline
+#endif
+
+TB_HD uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+TB_HD uint32_t perm_key(uint32_t seed, int iter, int frame, int slice) {
+  uint32_t k = seed ^ 0x5bd1e995U;
+  k = mix32(k ^ ((uint32_t)iter * 0x9e3779b1U));
+  k = mix32(k ^ ((uint32_t)frame * 0x85ebca77U));
+  k = mix32(k ^ ((uint32_t)slice * 0xc2b2ae3dU));
+  return k;
+}
+
+TB_HD uint32_t perm(uint32_t n, uint32_t N, uint32_t key) {
+  uint32_t bits = 1;
+  while ((1u << bits) < N) bits++;
+  uint32_t h = (bits + 1) / 2;
+  uint32_t mask = (1u << h) - 1u;
+  do {
+    uint32_t L = n >> h, R = n & mask;
+    for (uint32_t r = 0; r < 4; r++) {
+      uint32_t F = mix32(R ^ (key + r * 0x9e3779b9U)) & mask;
+      uint32_t nl = R;
+      R = L ^ F;
+      L = nl;
+    }
+    n = (L << h) | R;
+  } while (n >= N);
+  return n;
+}
+
+}  // namespace tb

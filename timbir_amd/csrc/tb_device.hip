@@ -1,0 +1,911 @@
+// C ABI of libtimbir_hip: one multi-resolution stage resident on the GPU.
+// Declarations and the reference functions each entry point replaces are in
+// include/timbir_dev.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "timbir_dev.h"
+#include "tb_common.h"
+
+namespace tb {
+__global__ void k_transpose_in(KP, const float *, const float *, double);
+__global__ void k_to_ref_layout(KP, const float *, float *);
+__global__ void k_forward(KP, const int *, float *, int);
+template <int NT>
+__global__ void k_update_voxels(KP, const int2 *, int, int, int, int, int);
+__global__ void k_offset_partial(KP, int, int, double *, double *);
+__global__ void k_offset_finalize(KP, int, const double *, const double *, double *, double *);
+__global__ void k_offset_gamma(KP, const double *, const double *, double *);
+__global__ void k_offset_apply(KP, int, const double *, const double *, const double *);
+__global__ void k_offset_store(KP, int, const double *, const double *, const double *);
+__global__ void k_var_partial(KP, double, double *);
+__global__ void k_var_apply(KP, double, double);
+__global__ void k_cost_data(KP, double *);
+__global__ void k_cost_prior(KP, double, double, double, double, double, double, const double *,
+                             double, double *);
+__global__ void k_vsc_filter(KP, float *);
+__global__ void k_vsc_select(KP, const float *, int);
+__global__ void k_mag_sum(KP, double *);
+__global__ void k_fill_f64(double *, size_t, double);
+__global__ void k_fill_f32(float *, size_t, float);
+__global__ void k_fill_u64(unsigned long long *, size_t);
+}  // namespace tb
+
+using namespace tb;
+
+// ---------------------------------------------------------------------------
+// errors and options
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1};
+
+static int fail(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail("%s:%d: %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" const char *tb_last_error(void) { return g_err.c_str(); }
+extern "C" void tb_get_options(tb_options *o) { *o = g_opt; }
+extern "C" void tb_set_options(const tb_options *o) { g_opt = *o; }
+
+// ---------------------------------------------------------------------------
+// stage
+// ---------------------------------------------------------------------------
+struct tb_stage {
+  // geometry and model (host)
+  int N_p, N_r, N_t, N_x, N_y, N_z, N_time, mult_xy, mult_z, U;
+  double delta_r, delta_t, R0, RMax, T0, x0, y0, delta_xy, delta_z, OffsetR, radius_obj;
+  double sSq, sSqp, sTq, sTqp, c_s, c_t, sfilt[27], tfilt, alpha, T, delta, var_est;
+  double convg_thresh, delta_recon;
+  int remove_rings, quad_convex, stage_idx, updateProjOffset, updateVar, initICD, initMagUpMap;
+  std::vector<int> view_off, view_idx, view_frame;
+  std::vector<float> angles;
+  int max_nv;
+  // device
+  hipStream_t stream = nullptr;
+  float *e = nullptr, *w = nullptr, *y = nullptr, *d = nullptr, *obj = nullptr, *mag = nullptr;
+  float *hr = nullptr, *filt = nullptr;
+  double *cosv = nullptr, *sinv = nullptr, *sfilt_d = nullptr;
+  int *view_off_d = nullptr, *view_idx_d = nullptr, *view_frame_d = nullptr;
+  uint8_t *mask = nullptr;
+  int *nh_list = nullptr, *nh_count = nullptr;
+  int nh_cap = 0;
+  int2 *cells[2] = {nullptr, nullptr};
+  int ncells[2] = {0, 0};
+  double *cell_voxmag = nullptr;
+  unsigned long long *cell_upd = nullptr, *cell_zero = nullptr, *cell_ent = nullptr;
+  double *off_pnum = nullptr, *off_pden = nullptr, *off_num = nullptr, *off_den = nullptr;
+  double *off_gamma = nullptr;
+  int off_chunks = 1, off_vpc = 1;
+  double *red = nullptr, *red_h = nullptr;
+  int red_n = 1024;
+  // stats
+  std::vector<hipEvent_t> ev;
+  int ev_used = 0;
+  double voxel_ms = 0, other_ms = 0;
+  long long voxel_launches = 0;
+  hipEvent_t o0 = nullptr, o1 = nullptr;
+
+  KP kp() const {
+    KP P;
+    memset(&P, 0, sizeof(P));
+    P.N_p = N_p; P.N_r = N_r; P.N_t = N_t; P.N_x = N_x; P.N_y = N_y; P.N_z = N_z;
+    P.N_time = N_time; P.mz = mult_z; P.K = kNhicdK;
+    P.R0 = R0; P.RMax = RMax; P.delta_r = delta_r; P.x0 = x0; P.y0 = y0;
+    P.delta_xy = delta_xy; P.OffsetR = OffsetR;
+    P.sSq = (float)sSq; P.sSqp = (float)sSqp; P.sTq = (float)sTq; P.sTqp = (float)sTqp;
+    P.c_s = (float)c_s; P.c_t = (float)c_t; P.tfilt = (float)tfilt; P.alpha = (float)alpha;
+    for (int i = 0; i < 27; i++) P.sfilt[i] = (float)sfilt[i];
+    P.T = (float)T; P.Td = (float)(T * delta); P.T_d = T; P.Td_d = T * delta; P.dl_d = delta;
+    P.e = e; P.w = w; P.y = y; P.d = d; P.obj = obj; P.mag = mag; P.hr = hr;
+    P.cosv = cosv; P.sinv = sinv; P.view_off = view_off_d; P.view_idx = view_idx_d;
+    P.mask = mask; P.nh_list = nh_list; P.nh_count = nh_count; P.nh_cap = nh_cap;
+    P.cell_voxmag = cell_voxmag; P.cell_upd = cell_upd; P.cell_zero = cell_zero;
+    P.cell_ent = cell_ent; P.order_seed = g_opt.order_seed;
+    return P;
+  }
+  size_t nsino() const { return (size_t)N_p * N_r * N_t; }
+  size_t nobj() const { return (size_t)N_time * N_z * N_y * N_x; }
+  size_t nxy() const { return (size_t)N_x * N_y; }
+};
+
+static int grid_for(size_t n, int block = 256) {
+  size_t g = (n + block - 1) / block;
+  return (int)std::min<size_t>(std::max<size_t>(g, 1), 65535 * 8);
+}
+
+// ---------------------------------------------------------------------------
+// host restatements of the stage set-up
+// ---------------------------------------------------------------------------
+// XT_Profile.c:47-176: trapezoid voxel profile convolved with the normalised
+// Hamming beam, sampled every OffsetR, one row of 65 values per view.
+static void detector_response(const float *angles, int N_p, double delta_xy, double OffsetR,
+                              double beam_width, float *H_r) {
+  std::vector<float> prof(kProfileRes);
+  double beam[kBeamRes], bsum = 0;
+  for (int i = 0; i < kBeamRes; i++) {
+    beam[i] = 0.54 - 0.46 * cos((2 * kPi / kBeamRes) * i);
+    bsum += beam[i];
+  }
+  for (int i = 0; i < kBeamRes; i++) beam[i] /= bsum;
+  const double r0 = -beam_width / 2, step = beam_width / kBeamRes;
+  const double tconst = kProfileRes / (2 * delta_xy);
+  for (int k = 0; k < N_p; k++) {
+    double ang = angles[k];
+    while (ang > kPi2) ang -= kPi2;
+    while (ang < 0) ang += kPi2;
+    const double peak = (ang <= kPi4) ? delta_xy / cos(ang) : delta_xy / cos(kPi2 - ang);
+    const double c45 = cos(kPi4);
+    const double d1 = c45 * cos(kPi4 - ang), d2 = c45 * fabs(cos(kPi4 + ang));
+    const double lc = 1 - d1, ln = 1 - d2, rn = 1 + d2, rc = 1 + d1;
+    for (int j = 0; j < kProfileRes; j++) {
+      const double t = 2.0 * j / kProfileRes;
+      double v;
+      if (t <= lc || t >= rc) v = 0;
+      else if (t > rn) v = peak * (rc - t) / (rc - rn);
+      else if (t >= ln) v = peak;
+      else v = peak * (t - lc) / (ln - lc);
+      prof[j] = (float)v;
+    }
+    for (int i = 0; i < kRespBins; i++) {
+      const double rmin = i * OffsetR - delta_xy;
+      double s = 0;
+      for (int q = 0; q < kBeamRes; q++) {
+        const double r = r0 + q * step;
+        const int idx = (int)floor((r - rmin) * tconst);
+        if (idx >= 0 && idx <= kProfileRes - 1) s += prof[idx] * beam[q];
+      }
+      H_r[(size_t)k * kHrStride + i] = (float)s;
+    }
+    H_r[(size_t)k * kHrStride + kRespBins] = 0.0f;
+  }
+}
+
+extern "C" void tb_stage_destroy(tb_stage *st) {
+  if (!st) return;
+  if (st->stream) (void)hipStreamSynchronize(st->stream);
+  void *bufs[] = {st->e, st->w, st->y, st->d, st->obj, st->mag, st->hr, st->filt, st->cosv,
+                  st->sinv, st->sfilt_d, st->view_off_d, st->view_idx_d, st->view_frame_d,
+                  st->mask, st->nh_list, st->nh_count, st->cells[0], st->cells[1],
+                  st->cell_voxmag, st->cell_upd, st->cell_zero, st->cell_ent, st->off_pnum,
+                  st->off_pden, st->off_num, st->off_den, st->off_gamma, st->red};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (st->red_h) (void)hipHostFree(st->red_h);
+  for (auto &e : st->ev) (void)hipEventDestroy(e);
+  if (st->o0) (void)hipEventDestroy(st->o0);
+  if (st->o1) (void)hipEventDestroy(st->o1);
+  if (st->stream) (void)hipStreamDestroy(st->stream);
+  delete st;
+}
+
+template <class T>
+static hipError_t dalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+static int stage_create_impl(tb_stage *st, const float *projections, const float *weights,
+                             const float *proj_angles, const float *proj_times,
+                             const float *recon_times, int proj_rows, int proj_cols,
+                             int proj_num, int recon_num, float vox_wid, float rot_center,
+                             float sig_s, float sig_t, float c_s, float c_t, float convg_thresh,
+                             int remove_rings, int quad_convex, float huber_delta, float huber_T,
+                             int mult_xy, int mult_z, int stage_idx, int prev_mult_z,
+                             double var_est) {
+  if (proj_rows <= 0 || proj_cols <= 0 || proj_num <= 0 || recon_num <= 0)
+    return fail("tb_stage_create: non-positive size");
+  if (mult_xy <= 0 || mult_z <= 0 || proj_cols % mult_xy || proj_rows % mult_z)
+    return fail("tb_stage_create: mult_xy=%d mult_z=%d do not divide %d x %d", mult_xy, mult_z,
+                proj_cols, proj_rows);
+  // initStatStructures (XT_Init.c:449-669), one process
+  const double Length_R = (double)vox_wid * proj_cols, Length_T = (double)vox_wid * proj_rows;
+  st->N_p = proj_num; st->N_r = proj_cols; st->N_t = proj_rows; st->N_time = recon_num;
+  st->convg_thresh = convg_thresh; st->alpha = kAlpha;
+  st->remove_rings = remove_rings; st->quad_convex = quad_convex;
+  st->T = quad_convex == 0 ? (double)huber_T : kZingerOffT;
+  st->delta = quad_convex == 0 ? (double)huber_delta : kZingerOffDelta;
+  st->delta_r = Length_R / st->N_r;
+  st->delta_t = Length_T / st->N_t;
+  st->R0 = -(double)rot_center * st->delta_r;
+  st->RMax = (st->N_r - (double)rot_center) * st->delta_r;
+  st->T0 = -Length_T / 2.0;
+  st->x0 = st->R0;
+  st->y0 = -Length_R / 2.0;
+  st->radius_obj = (double)vox_wid * proj_cols / 2;
+  // view -> frame map (XT_Init.c:69-131): half-open [rt[i], rt[i+1])
+  for (int i = 0; i < proj_num - 1; i++)
+    if (proj_times[i + 1] < proj_times[i])
+      return fail("Projection acquisition time decreased for increasing index.");
+  for (int i = 0; i < recon_num; i++)
+    if (recon_times[i + 1] < recon_times[i])
+      return fail("Reconstruction times decreased for increasing index.");
+  st->view_off.assign(recon_num + 1, 0);
+  st->view_frame.assign(proj_num, -1);
+  st->view_idx.clear();
+  int start = 0;
+  for (int i = 0; i < recon_num; i++) {
+    st->view_off[i] = (int)st->view_idx.size();
+    for (int j = start; j < proj_num; j++) {
+      const double t = proj_times[j];
+      if (t >= recon_times[i] && t < recon_times[i + 1]) {
+        st->view_idx.push_back(j);
+        st->view_frame[j] = i;
+      }
+      if (t >= recon_times[i + 1]) {
+        start = j;
+        break;
+      }
+    }
+  }
+  st->view_off[recon_num] = (int)st->view_idx.size();
+  st->delta_recon = (double)st->view_idx.size() / recon_num;
+  st->max_nv = 0;
+  for (int i = 0; i < recon_num; i++)
+    st->max_nv = std::max(st->max_nv, st->view_off[i + 1] - st->view_off[i]);
+  // initDynStructures (XT_Init.c:211-445)
+  st->stage_idx = stage_idx; st->mult_xy = mult_xy; st->mult_z = mult_z;
+  if (stage_idx == 0) st->initICD = 0;
+  else if (mult_z == prev_mult_z) st->initICD = 2;
+  else if (prev_mult_z == 2 * mult_z) st->initICD = 3;
+  else return fail("Multi-resolution scaling is not supported");
+  st->updateProjOffset = stage_idx == 0 ? 0 : (stage_idx == 1 ? 2 : 3);
+  if (remove_rings == 0) st->updateProjOffset = 0;
+  st->initMagUpMap = stage_idx > 0;
+  st->updateVar = stage_idx > 0 && quad_convex == 0;
+  st->var_est = var_est;
+  st->N_x = st->N_r / mult_xy; st->N_y = st->N_r / mult_xy; st->N_z = st->N_t / mult_z;
+  st->delta_xy = mult_xy * st->delta_r;
+  st->delta_z = mult_z * st->delta_t;
+  st->OffsetR = (st->delta_xy / sqrt(2.0) + st->delta_r / 2.0) / kRespBins;
+  st->sSq = pow(sig_s * (double)mult_xy, kMrfQ);
+  st->sSqp = pow(sig_s * (double)mult_xy, kMrfQ - kMrfP);
+  st->sTq = pow(sig_t * st->delta_recon, kMrfQ);
+  st->sTqp = pow(sig_t * st->delta_recon, kMrfQ - kMrfP);
+  st->c_s = c_s; st->c_t = c_t;
+  // initFilter (XT_Init.c:144-190): inverse-distance weights normalised so
+  // the 26 spatial and 2 temporal weights sum to mult_xy^3 * delta_recon
+  {
+    const double prior_const = (double)mult_xy * mult_xy * mult_xy * st->delta_recon;
+    double sum = 2.0;
+    for (int i = 0; i < 27; i++) {
+      const int a = i / 9 - 1, b = (i / 3) % 3 - 1, c = i % 3 - 1;
+      const double d2 = a * a + b * b + c * c;
+      st->sfilt[i] = d2 > 0 ? 1.0 / sqrt(d2) : 0.0;
+      sum += st->sfilt[i];
+    }
+    for (int i = 0; i < 27; i++) st->sfilt[i] = prior_const * st->sfilt[i] / sum;
+    st->tfilt = prior_const * 1.0 / sum;
+  }
+  // mask (XT_ICD_update.c:1140-1149)
+  std::vector<uint8_t> mask_h(st->nxy());
+  st->U = 0;
+  for (int j = 0; j < st->N_y; j++)
+    for (int k = 0; k < st->N_x; k++) {
+      const double x = st->x0 + ((double)k + 0.5) * st->delta_xy;
+      const double yv = st->y0 + ((double)j + 0.5) * st->delta_xy;
+      const uint8_t m = (x * x + yv * yv < st->radius_obj * st->radius_obj) ? 1 : 0;
+      mask_h[(size_t)j * st->N_x + k] = m;
+      st->U += m;
+    }
+  std::vector<float> hr_h((size_t)st->N_p * kHrStride);
+  detector_response(proj_angles, st->N_p, st->delta_xy, st->OffsetR, st->delta_r, hr_h.data());
+  std::vector<double> cs(st->N_p), sn(st->N_p);
+  for (int i = 0; i < st->N_p; i++) {  // calculateSinCos (XT_Init.c:193-204)
+    cs[i] = cos((double)proj_angles[i]);
+    sn[i] = sin((double)proj_angles[i]);
+  }
+  st->angles.assign(proj_angles, proj_angles + proj_num);
+
+  // device buffers
+  HIPCHK(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
+  const size_t ns = st->nsino(), no = st->nobj(), nxy = st->nxy();
+  const int ncell = st->N_time * st->N_z;
+  HIPCHK(dalloc(&st->e, ns));
+  HIPCHK(dalloc(&st->w, ns));
+  HIPCHK(dalloc(&st->y, ns));
+  HIPCHK(dalloc(&st->d, (size_t)st->N_r * st->N_t));
+  HIPCHK(dalloc(&st->obj, no));
+  HIPCHK(dalloc(&st->mag, no));
+  HIPCHK(dalloc(&st->filt, no));
+  HIPCHK(dalloc(&st->hr, hr_h.size()));
+  HIPCHK(dalloc(&st->cosv, st->N_p));
+  HIPCHK(dalloc(&st->sinv, st->N_p));
+  HIPCHK(dalloc(&st->sfilt_d, 27));
+  HIPCHK(dalloc(&st->view_off_d, st->view_off.size()));
+  HIPCHK(dalloc(&st->view_idx_d, st->view_idx.size()));
+  HIPCHK(dalloc(&st->view_frame_d, st->view_frame.size()));
+  HIPCHK(dalloc(&st->mask, nxy));
+  st->nh_cap = std::max(1, st->U / kNhicdK);
+  HIPCHK(dalloc(&st->nh_list, (size_t)ncell * st->nh_cap));
+  HIPCHK(dalloc(&st->nh_count, ncell));
+  HIPCHK(dalloc(&st->cell_voxmag, ncell));
+  HIPCHK(dalloc(&st->cell_upd, ncell));
+  HIPCHK(dalloc(&st->cell_zero, ncell));
+  HIPCHK(dalloc(&st->cell_ent, ncell));
+  const int npix = st->N_r * st->N_t;
+  st->off_vpc = std::max(1, std::min(st->N_p, 64));
+  st->off_chunks = (st->N_p + st->off_vpc - 1) / st->off_vpc;
+  HIPCHK(dalloc(&st->off_pnum, (size_t)npix * st->off_chunks));
+  HIPCHK(dalloc(&st->off_pden, (size_t)npix * st->off_chunks));
+  HIPCHK(dalloc(&st->off_num, npix));
+  HIPCHK(dalloc(&st->off_den, npix));
+  HIPCHK(dalloc(&st->off_gamma, 1));
+  HIPCHK(dalloc(&st->red, st->red_n));
+  HIPCHK(hipHostMalloc((void **)&st->red_h, st->red_n * sizeof(double), hipHostMallocDefault));
+  // cells of each colour: (f + z) & 1
+  for (int c = 0; c < 2; c++) {
+    std::vector<int2> cl;
+    for (int f = 0; f < st->N_time; f++)
+      for (int z = 0; z < st->N_z; z++)
+        if (((f + z) & 1) == c) cl.push_back(make_int2(f, z));
+    st->ncells[c] = (int)cl.size();
+    HIPCHK(dalloc(&st->cells[c], cl.size()));
+    if (!cl.empty())
+      HIPCHK(hipMemcpyAsync(st->cells[c], cl.data(), cl.size() * sizeof(int2),
+                            hipMemcpyHostToDevice, st->stream));
+  }
+  HIPCHK(hipMemcpyAsync(st->hr, hr_h.data(), hr_h.size() * sizeof(float), hipMemcpyHostToDevice,
+                        st->stream));
+  HIPCHK(hipMemcpyAsync(st->cosv, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice,
+                        st->stream));
+  HIPCHK(hipMemcpyAsync(st->sinv, sn.data(), sn.size() * sizeof(double), hipMemcpyHostToDevice,
+                        st->stream));
+  HIPCHK(hipMemcpyAsync(st->sfilt_d, st->sfilt, sizeof(st->sfilt), hipMemcpyHostToDevice,
+                        st->stream));
+  HIPCHK(hipMemcpyAsync(st->view_off_d, st->view_off.data(), st->view_off.size() * sizeof(int),
+                        hipMemcpyHostToDevice, st->stream));
+  if (!st->view_idx.empty())
+    HIPCHK(hipMemcpyAsync(st->view_idx_d, st->view_idx.data(), st->view_idx.size() * sizeof(int),
+                          hipMemcpyHostToDevice, st->stream));
+  HIPCHK(hipMemcpyAsync(st->view_frame_d, st->view_frame.data(),
+                        st->view_frame.size() * sizeof(int), hipMemcpyHostToDevice, st->stream));
+  HIPCHK(hipMemcpyAsync(st->mask, mask_h.data(), nxy, hipMemcpyHostToDevice, st->stream));
+  // raw scan -> device layout
+  float *raw_p = nullptr, *raw_w = nullptr;
+  HIPCHK(dalloc(&raw_p, ns));
+  HIPCHK(dalloc(&raw_w, ns));
+  HIPCHK(hipMemcpyAsync(raw_p, projections, ns * sizeof(float), hipMemcpyHostToDevice,
+                        st->stream));
+  HIPCHK(hipMemcpyAsync(raw_w, weights, ns * sizeof(float), hipMemcpyHostToDevice, st->stream));
+  KP P = st->kp();
+  hipLaunchKernelGGL(k_transpose_in, dim3(grid_for(ns)), dim3(256), 0, st->stream, P, raw_p,
+                     raw_w, var_est);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(no)), dim3(256), 0, st->stream, st->obj, no, 0.f);
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(no)), dim3(256), 0, st->stream, st->mag, no, 0.f);
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(npix)), dim3(256), 0, st->stream, st->d,
+                     (size_t)npix, 0.f);
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(ncell)), dim3(256), 0, st->stream,
+                     st->cell_voxmag, (size_t)ncell, 0.0);
+  // no NHICD lines until the first selection (all-zero bits = int 0)
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(ncell)), dim3(256), 0, st->stream,
+                     reinterpret_cast<float *>(st->nh_count), (size_t)ncell, 0.f);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_upd,
+                     (size_t)ncell);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_zero,
+                     (size_t)ncell);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_ent,
+                     (size_t)ncell);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st->stream));
+  HIPCHK(hipFree(raw_p));
+  HIPCHK(hipFree(raw_w));
+  st->ev.resize(256);
+  for (auto &e : st->ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventCreate(&st->o0));
+  HIPCHK(hipEventCreate(&st->o1));
+  return 0;
+}
+
+extern "C" int tb_stage_create(tb_stage **out, const float *projections, const float *weights,
+                               const float *proj_angles, const float *proj_times,
+                               const float *recon_times, int32_t proj_rows, int32_t proj_cols,
+                               int32_t proj_num, int32_t recon_num, float vox_wid,
+                               float rot_center, float sig_s, float sig_t, float c_s, float c_t,
+                               float convg_thresh, int32_t remove_rings, int32_t quad_convex,
+                               float huber_delta, float huber_T, int32_t mult_xy,
+                               int32_t mult_z, int32_t stage_idx, int32_t prev_mult_z,
+                               double var_est) {
+  *out = nullptr;
+  tb_stage *st = new tb_stage();
+  int rc = stage_create_impl(st, projections, weights, proj_angles, proj_times, recon_times,
+                             proj_rows, proj_cols, proj_num, recon_num, vox_wid, rot_center,
+                             sig_s, sig_t, c_s, c_t, convg_thresh, remove_rings, quad_convex,
+                             huber_delta, huber_T, mult_xy, mult_z, stage_idx, prev_mult_z,
+                             var_est);
+  if (rc) {
+    std::string keep = g_err;
+    tb_stage_destroy(st);
+    g_err = keep;
+    return -1;
+  }
+  *out = st;
+  return 0;
+}
+
+extern "C" int tb_stage_info(const tb_stage *st, int32_t *ints, double *d) {
+  int32_t iv[] = {st->N_p, st->N_r, st->N_t, st->N_x, st->N_y, st->N_z, st->N_time,
+                  st->mult_xy, st->mult_z, st->N_z, st->updateProjOffset, st->updateVar,
+                  st->initICD, st->U};
+  memcpy(ints, iv, sizeof(iv));
+  double dv[] = {st->delta_r, st->delta_t, st->R0, st->RMax, st->T0, st->x0, st->y0,
+                 st->delta_xy, st->delta_z, st->OffsetR, st->radius_obj, st->sSq, st->sSqp,
+                 st->sTq, st->sTqp, st->c_s, st->c_t, st->tfilt, st->alpha, st->T, st->delta,
+                 st->var_est, st->delta_recon};
+  memcpy(d, dv, sizeof(dv));
+  memcpy(d + sizeof(dv) / sizeof(double), st->sfilt, sizeof(st->sfilt));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// transfers
+// ---------------------------------------------------------------------------
+static int h2d(tb_stage *st, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st->stream));
+  HIPCHK(hipStreamSynchronize(st->stream));
+  return 0;
+}
+static int d2h(tb_stage *st, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st->stream));
+  HIPCHK(hipStreamSynchronize(st->stream));
+  return 0;
+}
+
+extern "C" int tb_set_object(tb_stage *st, const float *o) {
+  return h2d(st, st->obj, o, st->nobj() * sizeof(float));
+}
+extern "C" int tb_get_object(tb_stage *st, float *o) {
+  return d2h(st, o, st->obj, st->nobj() * sizeof(float));
+}
+extern "C" int tb_set_offset(tb_stage *st, const float *d) {
+  return h2d(st, st->d, d, (size_t)st->N_r * st->N_t * sizeof(float));
+}
+extern "C" int tb_get_offset(tb_stage *st, float *d) {
+  return d2h(st, d, st->d, (size_t)st->N_r * st->N_t * sizeof(float));
+}
+extern "C" int tb_set_magmap(tb_stage *st, const float *m) {
+  return h2d(st, st->mag, m, st->nobj() * sizeof(float));
+}
+extern "C" int tb_get_magmap(tb_stage *st, float *m) {
+  return d2h(st, m, st->mag, st->nobj() * sizeof(float));
+}
+extern "C" int tb_get_detector_response(tb_stage *st, float *h) {
+  return d2h(st, h, st->hr, (size_t)st->N_p * kHrStride * sizeof(float));
+}
+extern "C" double tb_get_var_est(const tb_stage *st) { return st->var_est; }
+
+static int sino_to_host(tb_stage *st, const float *src, float *dst) {
+  float *tmp = nullptr;
+  HIPCHK(dalloc(&tmp, st->nsino()));
+  KP P = st->kp();
+  hipLaunchKernelGGL(k_to_ref_layout, dim3(grid_for(st->nsino())), dim3(256), 0, st->stream, P,
+                     src, tmp);
+  HIPCHK(hipGetLastError());
+  int rc = d2h(st, dst, tmp, st->nsino() * sizeof(float));
+  (void)hipFree(tmp);
+  return rc;
+}
+extern "C" int tb_get_error_sino(tb_stage *st, float *e) { return sino_to_host(st, st->e, e); }
+extern "C" int tb_get_weights(tb_stage *st, float *w) { return sino_to_host(st, st->w, w); }
+
+// initObject with initICD 2 (x-y) or 3 (x-y-z), XT_ICD_update.c:323-651:
+// bilinear upsampling by 2; 2D intermediate rows are rounded to float as the
+// reference's Real_arr_t buffers are, 3D intermediates stay double.
+static void up2d(const float *in, int Ny, int Nx, bool float_mid, std::vector<double> &tmp,
+                 double *out) {
+  const int W = 2 * Nx;
+  tmp.assign((size_t)Ny * W, 0.0);
+  auto rnd = [&](double v) { return float_mid ? (double)(float)v : v; };
+  for (int j = 0; j < Ny; j++) {
+    const float *r = in + (size_t)j * Nx;
+    double *t = tmp.data() + (size_t)j * W;
+    t[0] = rnd(r[0]);
+    t[1] = rnd((3.0 * r[0] + r[1]) / 4.0);
+    t[W - 1] = rnd(r[Nx - 1]);
+    t[W - 2] = rnd((r[Nx - 2] + 3.0 * r[Nx - 1]) / 4.0);
+    for (int k = 1; k < Nx - 1; k++) {
+      t[2 * k] = rnd((r[k - 1] + 3.0 * r[k]) / 4.0);
+      t[2 * k + 1] = rnd((3.0 * r[k] + r[k + 1]) / 4.0);
+    }
+  }
+  const double *T = tmp.data();
+  for (int k = 0; k < W; k++) {
+    out[k] = T[k];
+    out[W + k] = (3.0 * T[k] + T[W + k]) / 4.0;
+    out[(size_t)(2 * Ny - 1) * W + k] = T[(size_t)(Ny - 1) * W + k];
+    out[(size_t)(2 * Ny - 2) * W + k] =
+        (T[(size_t)(Ny - 2) * W + k] + 3.0 * T[(size_t)(Ny - 1) * W + k]) / 4.0;
+  }
+  for (int j = 1; j < Ny - 1; j++)
+    for (int k = 0; k < W; k++) {
+      out[(size_t)(2 * j) * W + k] = (T[(size_t)(j - 1) * W + k] + 3.0 * T[(size_t)j * W + k]) / 4.0;
+      out[(size_t)(2 * j + 1) * W + k] =
+          (3 * T[(size_t)j * W + k] + T[(size_t)(j + 1) * W + k]) / 4.0;
+    }
+}
+
+static void upz(const double *in, int Nz, size_t P, double *out) {
+  for (size_t q = 0; q < P; q++) {
+    out[q] = in[q];
+    out[P + q] = (3.0 * in[q] + in[P + q]) / 4.0;
+    out[(size_t)(2 * Nz - 1) * P + q] = in[(size_t)(Nz - 1) * P + q];
+    out[(size_t)(2 * Nz - 2) * P + q] =
+        (3.0 * in[(size_t)(Nz - 1) * P + q] + in[(size_t)(Nz - 2) * P + q]) / 4.0;
+  }
+  for (int s = 1; s < Nz - 1; s++)
+    for (size_t q = 0; q < P; q++) {
+      out[(size_t)(2 * s) * P + q] = (in[(size_t)(s - 1) * P + q] + 3.0 * in[(size_t)s * P + q]) / 4.0;
+      out[(size_t)(2 * s + 1) * P + q] =
+          (3.0 * in[(size_t)s * P + q] + in[(size_t)(s + 1) * P + q]) / 4.0;
+    }
+}
+
+static void upsample(const float *prev, int prev_nz, int Ny, int Nx, int N_time, int out_nz,
+                     bool three_d, bool float_mid, std::vector<float> &out) {
+  const size_t P2 = (size_t)4 * Ny * Nx, P1 = (size_t)Ny * Nx;
+  out.assign((size_t)N_time * out_nz * P2, 0.f);
+  std::vector<double> tmp, up(P2 * prev_nz), uz(P2 * 2 * prev_nz);
+  for (int f = 0; f < N_time; f++) {
+    for (int s = 0; s < prev_nz; s++)
+      up2d(prev + ((size_t)f * prev_nz + s) * P1, Ny, Nx, float_mid && !three_d, tmp,
+           up.data() + (size_t)s * P2);
+    const double *src = up.data();
+    if (three_d) {
+      upz(up.data(), prev_nz, P2, uz.data());
+      src = uz.data();
+    }
+    for (size_t q = 0; q < (size_t)out_nz * P2; q++)
+      out[(size_t)f * out_nz * P2 + q] = (float)src[q];
+  }
+}
+
+extern "C" int tb_init_from_coarse(tb_stage *st, const float *prev_obj, int32_t prev_nz,
+                                   const float *prev_mag) {
+  if (st->initICD != 2 && st->initICD != 3) return fail("tb_init_from_coarse: stage 0");
+  const bool three = st->initICD == 3;
+  if ((three && prev_nz * 2 != st->N_z) || (!three && prev_nz != st->N_z))
+    return fail("tb_init_from_coarse: previous N_z %d incompatible with %d", prev_nz, st->N_z);
+  std::vector<float> o;
+  upsample(prev_obj, prev_nz, st->N_y / 2, st->N_x / 2, st->N_time, st->N_z, three, true, o);
+  if (tb_set_object(st, o.data())) return -1;
+  if (prev_mag && st->initMagUpMap) {
+    // one z-block per slice in both stages, so the block counts follow N_z
+    upsample(prev_mag, prev_nz, st->N_y / 2, st->N_x / 2, st->N_time, st->N_z, three, true, o);
+    if (tb_set_magmap(st, o.data())) return -1;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// timing helpers
+// ---------------------------------------------------------------------------
+static hipEvent_t *ev_pair(tb_stage *st) {
+  if (st->ev_used + 2 > (int)st->ev.size()) return nullptr;
+  hipEvent_t *p = &st->ev[st->ev_used];
+  st->ev_used += 2;
+  return p;
+}
+
+static int harvest_events(tb_stage *st) {
+  if (st->ev_used == 0) return 0;
+  HIPCHK(hipEventSynchronize(st->ev[st->ev_used - 1]));
+  for (int i = 0; i < st->ev_used; i += 2) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, st->ev[i], st->ev[i + 1]));
+    st->voxel_ms += ms;
+  }
+  st->ev_used = 0;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// operations
+// ---------------------------------------------------------------------------
+extern "C" int tb_init_error_sino(tb_stage *st) {
+  KP P = st->kp();
+  const size_t n = (size_t)st->N_z * st->N_p * st->N_r;
+  hipLaunchKernelGGL(k_forward, dim3(grid_for(n, 128)), dim3(128), 0, st->stream, P,
+                     st->view_frame_d, st->e, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st->stream));
+  return 0;
+}
+
+extern "C" int tb_forward_project(tb_stage *st, float *out) {
+  float *tmp = nullptr;
+  HIPCHK(dalloc(&tmp, st->nsino()));
+  KP P = st->kp();
+  const size_t n = (size_t)st->N_z * st->N_p * st->N_r;
+  hipLaunchKernelGGL(k_forward, dim3(grid_for(n, 128)), dim3(128), 0, st->stream, P,
+                     st->view_frame_d, tmp, 0);
+  HIPCHK(hipGetLastError());
+  int rc = sino_to_host(st, tmp, out);
+  (void)hipFree(tmp);
+  return rc;
+}
+
+static int reduce_partials(tb_stage *st, int nblocks, double *out) {
+  HIPCHK(hipMemcpyAsync(st->red_h, st->red, nblocks * sizeof(double), hipMemcpyDeviceToHost,
+                        st->stream));
+  HIPCHK(hipStreamSynchronize(st->stream));
+  double s = 0;
+  for (int i = 0; i < nblocks; i++) s += st->red_h[i];
+  *out = s;
+  return 0;
+}
+
+extern "C" int tb_cost(tb_stage *st, double *total, double *forward, double *prior) {
+  KP P = st->kp();
+  const int nb = std::min(st->red_n, grid_for(st->nsino()));
+  hipLaunchKernelGGL(k_cost_data, dim3(nb), dim3(256), 0, st->stream, P, st->red);
+  HIPCHK(hipGetLastError());
+  double fwd, pri;
+  if (reduce_partials(st, nb, &fwd)) return -1;
+  fwd /= 2.0;
+  const int nb2 = std::min(st->red_n, grid_for(st->nobj()));
+  hipLaunchKernelGGL(k_cost_prior, dim3(nb2), dim3(256), 0, st->stream, P, st->sSq, st->sSqp,
+                     st->c_s, st->sTq, st->sTqp, st->c_t, st->sfilt_d, st->tfilt, st->red);
+  HIPCHK(hipGetLastError());
+  if (reduce_partials(st, nb2, &pri)) return -1;
+  fwd += (double)st->N_p * (double)st->N_r * (double)st->N_t * log(st->var_est) / 2;
+  if (forward) *forward = fwd;
+  if (prior) *prior = pri;
+  if (total) *total = fwd + pri;
+  return 0;
+}
+
+static int offset_update_async(tb_stage *st, int type) {
+  if (type < 1 || type > 3) return fail("offset type %d", type);
+  KP P = st->kp();
+  const int npix = st->N_r * st->N_t;
+  hipLaunchKernelGGL(k_offset_partial, dim3(grid_for(npix, 64), st->off_chunks), dim3(64), 0,
+                     st->stream, P, type, st->off_vpc, st->off_pnum, st->off_pden);
+  hipLaunchKernelGGL(k_offset_finalize, dim3(grid_for(npix)), dim3(256), 0, st->stream, P,
+                     st->off_chunks, st->off_pnum, st->off_pden, st->off_num, st->off_den);
+  if (type == 2)
+    hipLaunchKernelGGL(k_offset_gamma, dim3(1), dim3(256), 0, st->stream, P, st->off_num,
+                       st->off_den, st->off_gamma);
+  hipLaunchKernelGGL(k_offset_apply, dim3(grid_for(st->nsino())), dim3(256), 0, st->stream, P,
+                     type, st->off_num, st->off_den, st->off_gamma);
+  hipLaunchKernelGGL(k_offset_store, dim3(grid_for(npix)), dim3(256), 0, st->stream, P, type,
+                     st->off_num, st->off_den, st->off_gamma);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int tb_offset_update(tb_stage *st, int32_t type) {
+  if (offset_update_async(st, type)) return -1;
+  HIPCHK(hipStreamSynchronize(st->stream));
+  return 0;
+}
+
+extern "C" int tb_variance_update(tb_stage *st) {
+  KP P = st->kp();
+  const int nb = std::min(st->red_n, grid_for(st->nsino()));
+  hipLaunchKernelGGL(k_var_partial, dim3(nb), dim3(256), 0, st->stream, P, st->var_est, st->red);
+  HIPCHK(hipGetLastError());
+  double acc;
+  if (reduce_partials(st, nb, &acc)) return -1;
+  const double vnew = acc / ((double)st->N_p * (double)st->N_r * (double)st->N_t);
+  hipLaunchKernelGGL(k_var_apply, dim3(grid_for(st->nsino())), dim3(256), 0, st->stream, P,
+                     st->var_est, vnew);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st->stream));
+  st->var_est = vnew;
+  return 0;
+}
+
+static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk,
+                         int seq, int pix, int count_mag) {
+  if (ncells == 0) return 0;
+  KP P = st->kp();
+  hipEvent_t *evp = ev_pair(st);
+  if (!evp) {
+    if (harvest_events(st)) return -1;
+    evp = ev_pair(st);
+  }
+  HIPCHK(hipEventRecord(evp[0], st->stream));
+  if (st->max_nv <= 64)
+    hipLaunchKernelGGL(k_update_voxels<64>, dim3(ncells), dim3(64), 0, st->stream, P, cells,
+                       iter, chunk, seq, pix, count_mag);
+  else if (st->max_nv <= 128)
+    hipLaunchKernelGGL(k_update_voxels<128>, dim3(ncells), dim3(128), 0, st->stream, P, cells,
+                       iter, chunk, seq, pix, count_mag);
+  else
+    hipLaunchKernelGGL(k_update_voxels<256>, dim3(ncells), dim3(256), 0, st->stream, P, cells,
+                       iter, chunk, seq, pix, count_mag);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(evp[1], st->stream));
+  st->voxel_launches++;
+  return 0;
+}
+
+extern "C" int tb_voxel_update(tb_stage *st, int32_t f, int32_t z, int32_t y, int32_t x,
+                               float *value) {
+  if (f < 0 || f >= st->N_time || z < 0 || z >= st->N_z || y < 0 || y >= st->N_y || x < 0 ||
+      x >= st->N_x)
+    return fail("tb_voxel_update: voxel out of range");
+  int2 c = make_int2(f, z);
+  int2 *dc = nullptr;
+  HIPCHK(dalloc(&dc, 1));
+  HIPCHK(hipMemcpyAsync(dc, &c, sizeof(c), hipMemcpyHostToDevice, st->stream));
+  const int pix = y * st->N_x + x;
+  if (launch_update(st, dc, 1, 1, 0, 2, pix, 0)) return -1;
+  HIPCHK(hipMemcpyAsync(value, st->obj + ((size_t)f * st->N_z + z) * st->nxy() + pix,
+                        sizeof(float), hipMemcpyDeviceToHost, st->stream));
+  HIPCHK(hipStreamSynchronize(st->stream));
+  (void)hipFree(dc);
+  return harvest_events(st);
+}
+
+extern "C" int tb_update_colour(tb_stage *st, int32_t colour, int32_t iter, int32_t chunk,
+                                int32_t nhicd) {
+  if (colour < 0 || colour > 1 || chunk < 0 || chunk >= kNhicdK)
+    return fail("tb_update_colour: bad colour/chunk");
+  if (launch_update(st, st->cells[colour], st->ncells[colour], iter, chunk, nhicd ? 1 : 0, 0,
+                    nhicd ? 0 : 1))
+    return -1;
+  HIPCHK(hipStreamSynchronize(st->stream));
+  return harvest_events(st);
+}
+
+static int vsc_async(tb_stage *st) {
+  KP P = st->kp();
+  hipLaunchKernelGGL(k_vsc_filter, dim3(grid_for(st->nobj())), dim3(256), 0, st->stream, P,
+                     st->filt);
+  // NHICDSelectNum = floor(UpdateSelectNum / NHICD_Iterations), XT_NHICD.c:182
+  hipLaunchKernelGGL(k_vsc_select, dim3(st->N_time * st->N_z), dim3(256), 0, st->stream, P,
+                     st->filt, st->U / kNhicdK);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int tb_vsc_select(tb_stage *st) {
+  if (vsc_async(st)) return -1;
+  HIPCHK(hipStreamSynchronize(st->stream));
+  return 0;
+}
+
+extern "C" int tb_nhicd_list(tb_stage *st, int32_t f, int32_t z, int32_t *xs, int32_t *ys) {
+  const int cell = f * st->N_z + z;
+  int n = 0;
+  if (d2h(st, &n, st->nh_count + cell, sizeof(int))) return -1;
+  std::vector<int> l(std::max(n, 1));
+  if (n && d2h(st, l.data(), st->nh_list + (size_t)cell * st->nh_cap, n * sizeof(int))) return -1;
+  for (int i = 0; i < n; i++) {
+    if (xs) xs[i] = l[i] % st->N_x;
+    if (ys) ys[i] = l[i] / st->N_x;
+  }
+  return n;
+}
+
+static int maybe_offset(tb_stage *st) {
+  if (st->updateProjOffset > 1) return offset_update_async(st, st->remove_rings);
+  return 0;
+}
+
+extern "C" int tb_icd_iteration(tb_stage *st, int32_t iter, int32_t *converged, double *ratio) {
+  const int ncell = st->N_time * st->N_z;
+  for (int j = 0; j < kNhicdK; j++) {
+    // total_vox_mag restarts every chunk (XT_ICD_update.c:976)
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(ncell)), dim3(256), 0, st->stream,
+                       st->cell_voxmag, (size_t)ncell, 0.0);
+    if (launch_update(st, st->cells[0], st->ncells[0], iter, j, 0, 0, 1)) return -1;
+    if (maybe_offset(st)) return -1;
+    if (launch_update(st, st->cells[1], st->ncells[1], iter, j, 0, 0, 1)) return -1;
+    if (maybe_offset(st)) return -1;
+    if (vsc_async(st)) return -1;
+    if (iter > 1 && g_opt.no_nhicd == 0) {
+      if (launch_update(st, st->cells[0], st->ncells[0], iter, j, 1, 0, 0)) return -1;
+      if (maybe_offset(st)) return -1;
+      if (launch_update(st, st->cells[1], st->ncells[1], iter, j, 1, 0, 0)) return -1;
+      if (maybe_offset(st)) return -1;
+    }
+  }
+  if (st->updateVar == 1 && tb_variance_update(st)) return -1;
+  // convergence rule (XT_ICD_update.c:1066-1104)
+  KP P = st->kp();
+  const int nb = std::min(st->red_n, grid_for(st->nobj()));
+  hipLaunchKernelGGL(k_mag_sum, dim3(nb), dim3(256), 0, st->stream, P, st->red);
+  HIPCHK(hipGetLastError());
+  double avg;
+  if (reduce_partials(st, nb, &avg)) return -1;
+  std::vector<double> vm(ncell);
+  if (d2h(st, vm.data(), st->cell_voxmag, ncell * sizeof(double))) return -1;
+  double vox_mag = 0;
+  for (double v : vm) vox_mag += v;
+  if (harvest_events(st)) return -1;
+  const double pct = 100 * avg / vox_mag;
+  if (ratio) *ratio = pct;
+  if (converged) *converged = pct < st->convg_thresh ? 1 : 0;
+  return 0;
+}
+
+extern "C" int tb_stage_run(tb_stage *st, int32_t max_iter, double *costs, int32_t *iters) {
+  if (max_iter <= 0) max_iter = g_opt.max_iterations > 0 ? g_opt.max_iterations : kMaxIters;
+  if (tb_init_error_sino(st)) return -1;
+  double last = 0;
+  const bool want_cost = g_opt.compute_cost || costs;
+  if (want_cost) {
+    if (tb_cost(st, &last, nullptr, nullptr)) return -1;
+    if (costs) costs[0] = last;
+  }
+  int it;
+  for (it = 1; it <= max_iter; it++) {
+    int conv = 0;
+    double ratio = 0;
+    if (tb_icd_iteration(st, it, &conv, &ratio)) return -1;
+    if (want_cost) {
+      double c;
+      if (tb_cost(st, &c, nullptr, nullptr)) return -1;
+      if (costs) costs[it] = c;
+      if (g_opt.compute_cost && g_opt.abort_on_cost_increase && c > last)
+        return fail("Cost value increased.");
+      last = c;
+    }
+    if (conv && it > 1) break;
+  }
+  if (iters) *iters = it > max_iter ? max_iter : it;
+  return 0;
+}
+
+extern "C" int tb_get_stats(tb_stage *st, tb_stats *out) {
+  if (harvest_events(st)) return -1;
+  const int ncell = st->N_time * st->N_z;
+  std::vector<unsigned long long> u(ncell), z(ncell), e(ncell);
+  if (d2h(st, u.data(), st->cell_upd, ncell * 8) || d2h(st, z.data(), st->cell_zero, ncell * 8) ||
+      d2h(st, e.data(), st->cell_ent, ncell * 8))
+    return -1;
+  memset(out, 0, sizeof(*out));
+  for (int i = 0; i < ncell; i++) {
+    out->updates += (int64_t)u[i];
+    out->zero_skips += (int64_t)z[i];
+    out->entries += (int64_t)e[i];
+  }
+  out->voxel_ms = st->voxel_ms;
+  out->other_ms = st->other_ms;
+  out->voxel_launches = st->voxel_launches;
+  out->alg_bytes = 12.0 * (double)out->entries;
+  return 0;
+}
+
+extern "C" int tb_reset_stats(tb_stage *st) {
+  if (harvest_events(st)) return -1;
+  const int ncell = st->N_time * st->N_z;
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_upd,
+                     (size_t)ncell);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_zero,
+                     (size_t)ncell);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_ent,
+                     (size_t)ncell);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st->stream));
+  st->voxel_ms = 0;
+  st->other_ms = 0;
+  st->voxel_launches = 0;
+  return 0;
+}
+
+extern "C" int tb_synchronize(tb_stage *st) {
+  HIPCHK(hipStreamSynchronize(st->stream));
+  return 0;
+}

@@ -1,0 +1,756 @@
+// HIP kernels of the MBIR_4D hot path for gfx950 (CDNA4).
+//
+// Cells: (frame f, slice z).  Voxels of different cells touch disjoint
+// error-sinogram entries (different frames own different views, different
+// slices own different detector rows: XT_VoxUpdate.c:62-65), and cells of
+// one colour (f + z) & 1 are never prior neighbours, so one colour's cells
+// are updated concurrently without atomics; inside a cell voxels are
+// updated one after another (Gauss-Seidel), as ICD requires.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "tb_common.h"
+
+namespace tb {
+
+__device__ __forceinline__ size_t sidx(const KP &P, int z, int v, int r, int m) {
+  return (((size_t)z * P.N_p + v) * P.N_r + r) * P.mz + m;
+}
+
+// A-matrix column of voxel (row, col) in one view: calcAMatrixColumnforAngle
+// (XT_AMatrix.c:50-110) in double with contraction off, so every bin and
+// value equals the reference's.  fn(bin, value) is called for each nonzero.
+template <class F>
+__device__ __forceinline__ void for_acol(const KP &P, const float *__restrict__ H, double c,
+                                         double s, int row, int col, F &&fn) {
+#pragma clang fp contract(off)
+  double x = P.x0 + ((double)col + 0.5) * P.delta_xy;
+  double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
+  double r = x * c - y * s;
+  double rmin = r - P.delta_xy, rmax = r + P.delta_xy;
+  if (rmax < P.R0 || rmin > P.RMax) return;
+  int lo = (int)floor((rmin - P.R0) / P.delta_r);
+  int hi = (int)floor((rmax - P.R0) / P.delta_r);
+  if (hi >= P.N_r) hi = P.N_r - 1;
+  if (lo < 0) lo = 0;
+  for (int j = lo; j <= hi; j++) {
+    double centre = P.R0 + ((double)j + 0.5) * P.delta_r;
+    double dist = fabs(r - centre);
+    int bin = (int)floor(dist / P.OffsetR);
+    if (bin >= 0 && bin < kRespBins) {
+      double w1 = dist - bin * P.OffsetR;
+      double w2 = P.OffsetR - w1;
+      double f1 = (w2 * (double)H[bin] + w1 * (double)H[bin + 1]) / P.OffsetR;
+      if (f1 > 0) fn(j, f1);
+    }
+  }
+}
+
+// Value of bin `j` of the column above, or 0 when j is outside it.
+__device__ __forceinline__ double acol_bin(const KP &P, const float *__restrict__ H, double c,
+                                           double s, int row, int col, int j) {
+#pragma clang fp contract(off)
+  double x = P.x0 + ((double)col + 0.5) * P.delta_xy;
+  double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
+  double r = x * c - y * s;
+  double rmin = r - P.delta_xy, rmax = r + P.delta_xy;
+  if (rmax < P.R0 || rmin > P.RMax) return 0.0;
+  int lo = (int)floor((rmin - P.R0) / P.delta_r);
+  int hi = (int)floor((rmax - P.R0) / P.delta_r);
+  if (hi >= P.N_r) hi = P.N_r - 1;
+  if (lo < 0) lo = 0;
+  if (j < lo || j > hi) return 0.0;
+  double centre = P.R0 + ((double)j + 0.5) * P.delta_r;
+  double dist = fabs(r - centre);
+  int bin = (int)floor(dist / P.OffsetR);
+  if (bin < 0 || bin >= kRespBins) return 0.0;
+  double w1 = dist - bin * P.OffsetR;
+  double w2 = P.OffsetR - w1;
+  double f1 = (w2 * (double)H[bin] + w1 * (double)H[bin + 1]) / P.OffsetR;
+  return f1 > 0 ? f1 : 0.0;
+}
+
+// ProjSelect b = |e sqrt(w)| < T evaluated as the reference does, in double
+// (XT_VoxUpdate.c:108-111); the float test decides all but a thin band.
+__device__ __forceinline__ bool selected(const KP &P, float e, float w) {
+  float u2 = e * e * w;
+  float T2 = P.T * P.T;
+  if (u2 < T2 * 0.99999f) return true;
+  if (u2 > T2 * 1.00001f) return false;
+  return fabs((double)e * sqrt((double)w)) < P.T_d;
+}
+
+// Neumaier compensated add.
+__device__ __forceinline__ void cadd(float &s, float &c, float x) {
+  float t = s + x;
+  if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
+
+// (s, c) pair sum over a wavefront with error-free TwoSum at each level.
+__device__ __forceinline__ void wave_csum(float &s, float &c) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    float so = __shfl_xor(s, off);
+    float co = __shfl_xor(c, off);
+    float t = s + so;
+    float bp = t - s;
+    float err = (s - (t - bp)) + (so - bp);
+    s = t;
+    c = c + co + err;
+  }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// qGGMRF rho'(d)/d for p = 1.2, q = 2 (XT_ICD_update.c:98-129): with q = 2
+// |d|^(q-2) = 1 and the d = 0 limit q/(sigma^q c) is the same expression.
+__device__ __forceinline__ float qggmrf_b(float d, float sq, float sqp, float c) {
+  float a = fabsf(d);
+  float t1 = powf(a, (float)(kMrfQ - kMrfP)) / sqp;
+  float t3 = c + t1;
+  return ((float)kMrfQ - (float)(kMrfQ - kMrfP) * t1 / t3) / (t3 * sq);
+}
+
+// ---------------------------------------------------------------------------
+// Input transpose: [view][row][col] -> [z][view][col][m], weights / var_est
+// (XT_Init.c:298-305, 327-330).
+// ---------------------------------------------------------------------------
+__global__ void k_transpose_in(KP P, const float *__restrict__ proj, const float *__restrict__ wgt,
+                               double var_est) {
+  size_t n = (size_t)P.N_p * P.N_t * P.N_r;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    int col = (int)(i % P.N_r);
+    int row = (int)((i / P.N_r) % P.N_t);
+    int v = (int)(i / ((size_t)P.N_r * P.N_t));
+    size_t o = sidx(P, row / P.mz, v, col, row % P.mz);
+    P.y[o] = proj[i];
+    P.w[o] = (float)((double)wgt[i] / var_est);
+  }
+}
+
+// Device layout -> reference layout [view][col][row] for outputs.
+__global__ void k_to_ref_layout(KP P, const float *__restrict__ src, float *__restrict__ dst) {
+  size_t n = (size_t)P.N_p * P.N_r * P.N_t;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    int t = (int)(i % P.N_t);
+    int r = (int)((i / P.N_t) % P.N_r);
+    int v = (int)(i / ((size_t)P.N_t * P.N_r));
+    dst[i] = src[sidx(P, t / P.mz, v, r, t % P.mz)];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection, gather form: thread per (slice z, view v, bin r) sums
+// the voxels of the frame that owns v whose column covers r, in raster
+// order, accumulating exactly as forward_project_voxel_AMat1D
+// (XT_ForwardProject.c:8-25) does: float += double(pixel) * double(A).
+// Every detector row of the slice receives the same value.
+// mode 0: out = A x; mode 1: e = y - A x - d.
+// ---------------------------------------------------------------------------
+__global__ void k_forward(KP P, const int *__restrict__ view_frame, float *__restrict__ out,
+                          int mode) {
+  size_t n = (size_t)P.N_z * P.N_p * P.N_r;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    int r = (int)(i % P.N_r);
+    int v = (int)((i / P.N_r) % P.N_p);
+    int z = (int)(i / ((size_t)P.N_r * P.N_p));
+    int f = view_frame[v];
+    float acc = 0.0f;
+    if (f >= 0) {
+      const float *H = P.hr + (size_t)v * kHrStride;
+      const double c = P.cosv[v], s = P.sinv[v];
+      const float *obj = P.obj + ((size_t)f * P.N_z + z) * P.N_x * P.N_y;
+      // columns whose projected centre lies in [lo, hi) can cover bin r
+      double lo = P.R0 + r * P.delta_r - P.delta_xy - 1e-9;
+      double hi = P.R0 + (r + 1) * P.delta_r + P.delta_xy + 1e-9;
+      for (int row = 0; row < P.N_y; row++) {
+        double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
+        int k0 = 0, k1 = P.N_x - 1;
+        if (fabs(c) > 1e-6) {
+          // x*c - y*s in [lo, hi)  =>  x in [(lo + y s)/c, (hi + y s)/c]
+          double xa = (lo + y * s) / c, xb = (hi + y * s) / c;
+          if (xa > xb) { double t = xa; xa = xb; xb = t; }
+          double ka = (xa - P.x0) / P.delta_xy - 0.5, kb = (xb - P.x0) / P.delta_xy - 0.5;
+          // clamp before converting so huge ranges cannot overflow an int
+          ka = fmin(fmax(ka, -4.0), (double)P.N_x + 4.0);
+          kb = fmin(fmax(kb, -4.0), (double)P.N_x + 4.0);
+          k0 = max(0, (int)floor(ka) - 1);
+          k1 = min(P.N_x - 1, (int)ceil(kb) + 1);
+        }
+        for (int col = k0; col <= k1; col++) {
+          float px = obj[(size_t)row * P.N_x + col];
+          double a = acol_bin(P, H, c, s, row, col, r);
+          if (a != 0.0) acc = (float)((double)acc + (double)px * a);
+        }
+      }
+    }
+    for (int m = 0; m < P.mz; m++) {
+      size_t o = sidx(P, z, v, r, m);
+      if (mode == 0) out[o] = acc;
+      else out[o] = (P.y[o] - acc) - P.d[(size_t)r * P.N_t + z * P.mz + m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Voxel update: one workgroup per cell of one colour, voxels in sequence.
+// Per voxel (compute_voxel_update_Atten, XT_VoxUpdate.c:43-115):
+//   threads stride over the frame's views, build the A column on the fly and
+//   accumulate theta1 = -sum A e w~, theta2 = sum A^2 w~ (Neumaier FP32);
+//   lanes 0..28 of wave 0 gather the 26 spatial + 2 temporal neighbours and
+//   form the qGGMRF surrogate sums (CE_FunctionalSubstitution :134-183);
+//   every thread solves x <- V + alpha (u - V) and subtracts A dx from its own
+//   views' entries.  A thread only ever touches its own views' entries, so
+//   the error-sinogram read-modify-write is thread-private.
+// seq: 0 HICD chunk of the keyed permutation, 1 NHICD list, 2 one pixel.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(NT) void k_update_voxels(KP P, const int2 *__restrict__ cells,
+                                                      int iter, int chunk, int seq, int pix1,
+                                                      int count_mag) {
+  constexpr int NW = NT / 64;
+  __shared__ float s_part[NW][4];
+  __shared__ float s_prior[2];
+  __shared__ float s_V;
+  __shared__ int s_skip;
+
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int Nxy = P.N_x * P.N_y;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+  float *magc = P.mag + (size_t)cell * Nxy;
+  float *objc = P.obj + (size_t)cell * Nxy;
+
+  int n0 = 0, n1 = 1;
+  uint32_t key = 0;
+  if (seq == 0) {
+    n0 = chunk * (Nxy / P.K);
+    n1 = (chunk == P.K - 1) ? Nxy : (chunk + 1) * (Nxy / P.K);
+    key = perm_key(P.order_seed, iter, f, z);
+  } else if (seq == 1) {
+    n1 = P.nh_count[cell];
+  }
+
+  double voxmag = 0.0;
+  unsigned long long nupd = 0, nzero = 0, nent = 0;
+
+  // neighbour slot of this lane (wave 0, lanes 0..28)
+  int ndz = 0, ndy = 0, ndx = 0, ndt = 0;
+  float ng = 0.0f;
+  bool nspatial = false;
+  if (wid == 0 && lane < 29) {
+    if (lane < 27) {
+      ndz = lane / 9 - 1;
+      ndy = (lane / 3) % 3 - 1;
+      ndx = lane % 3 - 1;
+      nspatial = lane != 13;
+      ng = P.sfilt[lane];
+    } else {
+      ndt = lane == 27 ? -1 : 1;
+      ng = P.tfilt;
+    }
+  }
+
+  for (int n = n0; n < n1; n++) {
+    int pix;
+    if (seq == 0) pix = (int)perm((uint32_t)n, (uint32_t)Nxy, key);
+    else if (seq == 1) pix = P.nh_list[(size_t)cell * P.nh_cap + n];
+    else pix = pix1;
+    const int yy = pix / P.N_x, xx = pix - yy * P.N_x;
+    if (P.mask[pix] != 1) {
+      if (seq == 1 && tid == 0) magc[pix] = 0.0f;  // XT_VoxUpdate.c:156
+      continue;
+    }
+
+    // neighbourhood (XT_VoxUpdate.c:168-233)
+    float nbv = 0.0f;
+    bool nbok = false;
+    if (wid == 0 && lane < 29) {
+      if (lane < 27) {
+        int zz = z + ndz, jj = yy + ndy, kk = xx + ndx;
+        if (zz >= 0 && zz < P.N_z && jj >= 0 && jj < P.N_y && kk >= 0 && kk < P.N_x) {
+          const float *q = &P.obj[(((size_t)f * P.N_z + zz) * P.N_y + jj) * P.N_x + kk];
+          // in-plane neighbours may have been written earlier in this launch
+          // by another lane: read them coherently from L2, not from the CU's L1
+          nbv = (ndz == 0) ? __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *q;
+          nbok = true;
+        }
+      } else {
+        int ff = f + ndt;
+        if (ff >= 0 && ff < P.N_time) {
+          nbv = P.obj[((size_t)ff * P.N_z + z) * Nxy + pix];
+          nbok = true;
+        }
+      }
+    }
+
+    // theta1 / theta2 over the frame's views
+    float t1s = 0.f, t1c = 0.f, t2s = 0.f, t2c = 0.f;
+    for (int q = tid; q < nv; q += NT) {
+      const int v = P.view_idx[vb + q];
+      const float *H = P.hr + (size_t)v * kHrStride;
+      for_acol(P, H, P.cosv[v], P.sinv[v], yy, xx, [&](int r, double ad) {
+        const float a = (float)ad;
+        for (int m = 0; m < P.mz; m++) {
+          const size_t s = sidx(P, z, v, r, m);
+          const float e = P.e[s], w = P.w[s];
+          const float cw = selected(P, e, w) ? w : P.Td * sqrtf(w) / fabsf(e);
+          cadd(t2s, t2c, a * a * cw);
+          cadd(t1s, t1c, -(e * a * cw));
+          nent++;
+        }
+      });
+    }
+    wave_csum(t1s, t1c);
+    wave_csum(t2s, t2c);
+
+    if (wid == 0) {
+      const float V = __shfl(nbv, 13);
+      // zero skipping (XT_VoxUpdate.c:236-259)
+      const bool pos = (lane < 29) && (lane != 13) && (nbv > 0.0f);
+      const bool anypos = __any(pos);
+      float pb = 0.f, pbx = 0.f;
+      if (lane < 29 && nbok && lane != 13) {
+        const float dlt = V - nbv;
+        const float b = (lane < 27) ? qggmrf_b(dlt, P.sSq, P.sSqp, P.c_s)
+                                    : qggmrf_b(dlt, P.sTq, P.sTqp, P.c_t);
+        pb = ng * b;
+        pbx = ng * b * (nbv - V);
+      }
+      pb = wave_sum(pb);
+      pbx = wave_sum(pbx);
+      if (lane == 0) {
+        s_prior[0] = pb;
+        s_prior[1] = pbx;
+        s_V = V;
+        s_skip = (V == 0.0f && iter > 1 && !anypos) ? 1 : 0;
+      }
+      (void)nspatial;
+    }
+    if (lane == 0) {
+      s_part[wid][0] = t1s;
+      s_part[wid][1] = t1c;
+      s_part[wid][2] = t2s;
+      s_part[wid][3] = t2c;
+    }
+    __syncthreads();
+
+    if (!s_skip) {
+      float T1 = 0.f, T1c = 0.f, T2 = 0.f, T2c = 0.f;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        cadd(T1, T1c, s_part[w][0]);
+        T1c += s_part[w][1];
+        cadd(T2, T2c, s_part[w][2]);
+        T2c += s_part[w][3];
+      }
+      const float th1 = T1 + T1c, th2 = T2 + T2c;
+      const float V = s_V;
+      // x = V + alpha (u - V), u - V = (sum g b (x_j - V) - theta1) / (sum g b + theta2)
+      const float step = P.alpha * (s_prior[1] - th1) / (s_prior[0] + th2);
+      const float xnew = V + step;
+      const float dx = xnew - V;
+      if (tid == 0) {
+        __hip_atomic_store(&objc[pix], xnew, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        magc[pix] = fabsf(dx);
+        if (count_mag) voxmag += fabs((double)xnew);
+        nupd++;
+      }
+      if (dx != 0.0f) {
+        for (int q = tid; q < nv; q += NT) {
+          const int v = P.view_idx[vb + q];
+          const float *H = P.hr + (size_t)v * kHrStride;
+          for_acol(P, H, P.cosv[v], P.sinv[v], yy, xx, [&](int r, double ad) {
+            const float a = (float)ad;
+            for (int m = 0; m < P.mz; m++) {
+              const size_t s = sidx(P, z, v, r, m);
+              P.e[s] = P.e[s] - a * dx;
+            }
+          });
+        }
+      }
+    } else {
+      if (tid == 0) {
+        magc[pix] = 0.0f;
+        nzero++;
+      }
+    }
+    __syncthreads();
+  }
+
+  // per-cell accounting
+  nent = (unsigned long long)wave_sum_d((double)nent);
+  __shared__ unsigned long long s_ent[NW];
+  if (lane == 0) s_ent[wid] = nent;
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long te = 0;
+    for (int w = 0; w < NW; w++) te += s_ent[w];
+    if (seq == 0 || seq == 1) {
+      P.cell_voxmag[cell] += voxmag;
+      P.cell_upd[cell] += nupd;
+      P.cell_zero[cell] += nzero;
+      P.cell_ent[cell] += te;
+    }
+  }
+}
+
+template __global__ void k_update_voxels<64>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_update_voxels<128>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_update_voxels<256>(KP, const int2 *, int, int, int, int, int);
+
+// ---------------------------------------------------------------------------
+// Offset passes (XT_ICD_update.c:753-917).  Per detector pixel (r, t) the
+// weights c = w (b) or T delta sqrt(w)/|e| (not b) are reduced over all views
+// in double: num = sum c e (type 1/2, with c*sign(e)*|e| for not-b) or
+// sum c (e + d) (type 3), den = sum c.  Partial sums over view chunks, then
+// a fixed-order finalize.
+// ---------------------------------------------------------------------------
+__global__ void k_offset_partial(KP P, int type, int views_per_chunk, double *__restrict__ pnum,
+                                 double *__restrict__ pden) {
+  const int npix = P.N_r * P.N_t;
+  const int chunk = blockIdx.y;
+  for (int px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    // px enumerates (z, r, m) so consecutive threads touch consecutive floats
+    const int m = px % P.mz;
+    const int r = (px / P.mz) % P.N_r;
+    const int z = px / (P.mz * P.N_r);
+    const int t = z * P.mz + m;
+    const double dof = P.d[(size_t)r * P.N_t + t];
+    double num = 0, den = 0;
+    const int v0 = chunk * views_per_chunk, v1 = min(P.N_p, v0 + views_per_chunk);
+    for (int v = v0; v < v1; v++) {
+      const size_t s = sidx(P, z, v, r, m);
+      const double e = P.e[s], w = P.w[s];
+      const double tmp = P.Td_d * sqrt(w);
+      const bool b = fabs(e * sqrt(w)) < P.T_d;
+      if (type == 3) {
+        if (b) { den += w; num += (e + dof) * w; }
+        else { den += tmp / fabs(e); num += (e + dof) * tmp / fabs(e); }
+      } else {
+        if (b) { num += e * w; den += w; }
+        else { double sg = (e > 0) - (e < 0); num += tmp * sg; den += tmp / fabs(e); }
+      }
+    }
+    pnum[(size_t)chunk * npix + px] = num;
+    pden[(size_t)chunk * npix + px] = den;
+  }
+}
+
+__global__ void k_offset_finalize(KP P, int nchunks, const double *__restrict__ pnum,
+                                  const double *__restrict__ pden, double *__restrict__ num,
+                                  double *__restrict__ den) {
+  const int npix = P.N_r * P.N_t;
+  for (int px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    double a = 0, b = 0;
+    for (int c = 0; c < nchunks; c++) {
+      a += pnum[(size_t)c * npix + px];
+      b += pden[(size_t)c * npix + px];
+    }
+    num[px] = a;
+    den[px] = b;
+  }
+}
+
+// type 2: gamma = sum(d + num/den) / sum(1/den), one workgroup
+__global__ void k_offset_gamma(KP P, const double *__restrict__ num, const double *__restrict__ den,
+                               double *__restrict__ gamma) {
+  const int npix = P.N_r * P.N_t;
+  double a = 0, b = 0;
+  for (int px = threadIdx.x; px < npix; px += blockDim.x) {
+    const int m = px % P.mz, r = (px / P.mz) % P.N_r, z = px / (P.mz * P.N_r);
+    const double dof = P.d[(size_t)r * P.N_t + z * P.mz + m];
+    a += dof + num[px] / den[px];
+    b += 1.0 / den[px];
+  }
+  __shared__ double sa[256], sb[256];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      sa[threadIdx.x] += sa[threadIdx.x + o];
+      sb[threadIdx.x] += sb[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *gamma = sa[0] / sb[0];
+}
+
+// apply: d and e for every view
+__global__ void k_offset_apply(KP P, int type, const double *__restrict__ num,
+                               const double *__restrict__ den, const double *__restrict__ gamma) {
+  const int npix = P.N_r * P.N_t;
+  const size_t n = (size_t)npix * P.N_p;
+  const double g = (type == 2) ? *gamma : 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i % P.mz);
+    const int r = (int)((i / P.mz) % P.N_r);
+    const int v = (int)((i / ((size_t)P.mz * P.N_r)) % P.N_p);
+    const int z = (int)(i / ((size_t)P.mz * P.N_r * P.N_p));
+    const int px = (z * P.N_r + r) * P.mz + m;
+    const size_t dofi = (size_t)r * P.N_t + z * P.mz + m;
+    const float dold = P.d[dofi];
+    float &e = P.e[i];
+    if (type == 3) {
+      const float x = (float)(num[px] / den[px]);
+      e += dold - x;
+    } else {
+      const double step = (num[px] - g) / den[px];
+      e = (float)((double)e - step);
+    }
+  }
+}
+
+// new offsets after k_offset_apply has consumed the old ones
+__global__ void k_offset_store(KP P, int type, const double *__restrict__ num,
+                               const double *__restrict__ den, const double *__restrict__ gamma) {
+  const int npix = P.N_r * P.N_t;
+  const double g = (type == 2) ? *gamma : 0.0;
+  for (int px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    const int m = px % P.mz, r = (px / P.mz) % P.N_r, z = px / (P.mz * P.N_r);
+    const size_t dofi = (size_t)r * P.N_t + z * P.mz + m;
+    if (type == 3) P.d[dofi] = (float)(num[px] / den[px]);
+    else P.d[dofi] = (float)((double)P.d[dofi] + (num[px] - g) / den[px]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Variance (XT_ICD_update.c:720-751) and cost (:186-319): FP64 partials per
+// workgroup, finished on the host in a fixed order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum_d(double v, double *sh) {
+  v = wave_sum_d(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += sh[i];
+  __syncthreads();
+  return t;
+}
+
+__global__ void k_var_partial(KP P, double var_old, double *__restrict__ part) {
+  __shared__ double sh[16];
+  const size_t n = (size_t)P.N_p * P.N_r * P.N_t;
+  double acc = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float wr = (float)((double)P.w[i] * var_old);
+    const double e = P.e[i], w = wr;
+    if (fabs(e * sqrt((double)P.w[i])) < P.T_d) acc += e * e * w;
+    else acc += fabs(e) * P.Td_d * sqrt(w * var_old);
+  }
+  double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_var_apply(KP P, double var_old, double var_new) {
+  const size_t n = (size_t)P.N_p * P.N_r * P.N_t;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float wr = (float)((double)P.w[i] * var_old);
+    P.w[i] = (float)((double)wr / var_new);
+  }
+}
+
+__global__ void k_cost_data(KP P, double *__restrict__ part) {
+  __shared__ double sh[16];
+  const size_t n = (size_t)P.N_p * P.N_r * P.N_t;
+  double acc = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double u = (double)P.e[i] * sqrt((double)P.w[i]);
+    if (fabs(u) < P.T_d) acc += u * u;
+    else acc += 2.0 * P.dl_d * P.T_d * fabs(u) + P.T_d * P.T_d * (1.0 - 2.0 * P.dl_d);
+  }
+  double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__device__ __forceinline__ double qggmrf_value_d(double d, double sq, double sqp, double c) {
+  double a = fabs(d);
+  return (a * a / sq) / (c + pow(a, kMrfQ - kMrfP) / sqp);
+}
+
+__global__ void k_cost_prior(KP P, double sSq, double sSqp, double cs, double sTq, double sTqp,
+                             double ct, const double *__restrict__ sfilt, double tfilt,
+                             double *__restrict__ part) {
+  __shared__ double sh[16];
+  const int Nxy = P.N_x * P.N_y;
+  const size_t n = (size_t)P.N_time * P.N_z * Nxy;
+  // the 13 forward neighbours in (dz, dy, dx) (XT_ICD_update.c:223-292)
+  const int fw[13][3] = {{0, 0, 1},  {0, 1, -1}, {0, 1, 0},  {0, 1, 1},  {1, -1, 0},
+                         {1, 0, 0},  {1, 1, 0},  {1, -1, -1}, {1, -1, 1}, {1, 0, -1},
+                         {1, 1, -1}, {1, 1, 1},  {1, 0, 1}};
+  double acc = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % P.N_x);
+    const int j = (int)((i / P.N_x) % P.N_y);
+    const int z = (int)((i / Nxy) % P.N_z);
+    const int f = (int)(i / ((size_t)Nxy * P.N_z));
+    const double v = P.obj[i];
+    for (int q = 0; q < 13; q++) {
+      const int zz = z + fw[q][0], jj = j + fw[q][1], kk = k + fw[q][2];
+      if (zz >= P.N_z || jj < 0 || jj >= P.N_y || kk < 0 || kk >= P.N_x) continue;
+      const double g = sfilt[(fw[q][0] + 1) * 9 + (fw[q][1] + 1) * 3 + (fw[q][2] + 1)];
+      const double o = P.obj[(((size_t)f * P.N_z + zz) * P.N_y + jj) * P.N_x + kk];
+      acc += g * qggmrf_value_d(v - o, sSq, sSqp, cs);
+    }
+    if (f + 1 < P.N_time)
+      acc += tfilt * qggmrf_value_d(v - (double)P.obj[i + (size_t)P.N_z * Nxy], sTq, sTqp, ct);
+  }
+  double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// ---------------------------------------------------------------------------
+// NHICD (XT_NHICD.c:128-216): 5x5 Hamming-weighted filter of each cell's
+// magnitude map, the k-th smallest value by 4-pass radix select, then the
+// pixels strictly above it in raster order.
+// ---------------------------------------------------------------------------
+__global__ void k_vsc_filter(KP P, float *__restrict__ filt) {
+  const double hw[5] = {0.0064, 0.0432, 0.0800, 0.0432, 0.0064};
+  const int Nxy = P.N_x * P.N_y;
+  const size_t n = (size_t)P.N_time * P.N_z * Nxy;
+  // the reference table is the outer product row x row / 0.08^2 ... but
+  // stored literally; keep its exact entries
+  const double tab[5][5] = {{0.0064, 0.0432, 0.0800, 0.0432, 0.0064},
+                            {0.0432, 0.2916, 0.5400, 0.2916, 0.0432},
+                            {0.0800, 0.5400, 1.0000, 0.5400, 0.0800},
+                            {0.0432, 0.2916, 0.5400, 0.2916, 0.0432},
+                            {0.0064, 0.0432, 0.0800, 0.0432, 0.0064}};
+  (void)hw;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % P.N_x);
+    const int j = (int)((i / P.N_x) % P.N_y);
+    const float *m = P.mag + (i / Nxy) * Nxy;
+    double acc = 0;
+    for (int a = -2; a <= 2; a++)
+      for (int c = -2; c <= 2; c++)
+        if (j + a >= 0 && j + a < P.N_y && k + c >= 0 && k + c < P.N_x)
+          acc += tab[a + 2][c + 2] * m[(size_t)(j + a) * P.N_x + (k + c)];
+    filt[i] = (float)acc;
+  }
+}
+
+// one workgroup (256 threads) per cell
+__global__ __launch_bounds__(256) void k_vsc_select(KP P, const float *__restrict__ filt,
+                                                    int want) {
+  const int cell = blockIdx.x;
+  const int Nxy = P.N_x * P.N_y;
+  const uint32_t *vals = reinterpret_cast<const uint32_t *>(filt + (size_t)cell * Nxy);
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t s_prefix, s_k;
+  // k-th smallest, 1-based (RandomizedSelect(A, 0, n-1, n - want))
+  if (threadIdx.x == 0) {
+    s_prefix = 0;
+    s_k = (uint32_t)(Nxy - want);
+  }
+  __syncthreads();
+  uint32_t thresh_bits;
+  if (want <= 0) {
+    thresh_bits = 0xffffffffu;  // nothing above
+  } else {
+    for (int pass = 0; pass < 4; pass++) {
+      const int shift = 24 - 8 * pass;
+      const uint32_t hmask = pass == 0 ? 0u : (0xffffffffu << (32 - 8 * pass));
+      hist[threadIdx.x] = 0;
+      __syncthreads();
+      const uint32_t pre = s_prefix;
+      for (int i = threadIdx.x; i < Nxy; i += blockDim.x) {
+        const uint32_t b = vals[i];
+        if ((b & hmask) == pre) atomicAdd(&hist[(b >> shift) & 0xff], 1u);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t k = s_k, acc = 0;
+        int bucket = 0;
+        for (; bucket < 256; bucket++) {
+          if (acc + hist[bucket] >= k) break;
+          acc += hist[bucket];
+        }
+        s_k = k - acc;
+        s_prefix = pre | ((uint32_t)bucket << shift);
+      }
+      __syncthreads();
+    }
+    thresh_bits = s_prefix;
+  }
+  const float thresh = __uint_as_float(thresh_bits);
+  // compaction in raster order: wave ballots + one cross-wave prefix
+  __shared__ int s_wcnt[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int base = 0;
+  for (int start = 0; start < Nxy; start += blockDim.x) {
+    const int i = start + threadIdx.x;
+    const bool flag = (want > 0) && i < Nxy && __uint_as_float(vals[i]) > thresh;
+    const unsigned long long bal = __ballot(flag);
+    const int inwave = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wcnt[wid] = __popcll(bal);
+    __syncthreads();
+    int off = base;
+    for (int w = 0; w < wid; w++) off += s_wcnt[w];
+    if (flag) {
+      const int pos = off + inwave;
+      if (pos < P.nh_cap) P.nh_list[(size_t)cell * P.nh_cap + pos] = i;
+    }
+    for (int w = 0; w < 4; w++) base += s_wcnt[w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P.nh_count[cell] = min(base, P.nh_cap);
+}
+
+// sum of the magnitude map over masked pixels (convergence numerator,
+// XT_ICD_update.c:1068-1072)
+__global__ void k_mag_sum(KP P, double *__restrict__ part) {
+  __shared__ double sh[16];
+  const int Nxy = P.N_x * P.N_y;
+  const size_t n = (size_t)P.N_time * P.N_z * Nxy;
+  double acc = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    if (P.mask[i % Nxy]) acc += P.mag[i];
+  double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void k_fill_f64(double *p, size_t n, double v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_f32(float *p, size_t n, float v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
+}  // namespace tb

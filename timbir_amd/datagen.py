@@ -141,6 +141,9 @@ def project(obj_slices: np.ndarray, angles: np.ndarray, n_bins: int, rot_center:
     """Line integrals of obj_slices [nz][N][N] for every angle ->
     [n_views][nz][n_bins] (reference input layout)."""
     nz, N, _ = obj_slices.shape
+    if nz > 1 and all(np.array_equal(obj_slices[0], obj_slices[s]) for s in range(1, nz)):
+        one = project(obj_slices[:1], angles, n_bins, rot_center, vox_wid, sub)
+        return np.repeat(one, nz, axis=1)
     off = (np.arange(sub) + 0.5) / sub
     k = (np.arange(N)[None, :, None, None] + off[None, None, None, :])
     i = (np.arange(N)[:, None, None, None] + off[None, None, :, None])
