@@ -64,6 +64,19 @@ def cpu_baseline(scan, stage_kw, threads: int, budget_s: float = 20.0):
                        f"{upd} voxel updates in {el:.1f} s")
 
 
+def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
+    """A plain device-to-device copy of nbytes, reps times: a counter run of
+    this command tells how FETCH_SIZE / WRITE_SIZE read for known traffic."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,10 +85,16 @@ def main():
     ap.add_argument("--small", action="store_true", help="tiny workload for a quick check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of known size (the scale for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
-                                   "(the z-slab multi-GPU split is described in DESIGN.md)"}))
+                                   "(a z-slab split over GPUs is designed, not built)"}))
         sys.exit(2)
 
     import timbir_amd as tb

@@ -30,7 +30,8 @@ extern "C" {
 typedef struct tb_stage tb_stage;
 
 /* Options the reference fixes with compile-time macros (XT_Constants.h,
- * src/MBIR_4D/Makefile:37).  Defaults: zeros except max_iterations = 1000. */
+ * src/MBIR_4D/Makefile:37).  Defaults: zeros except max_iterations = 1000,
+ * write_files = 1 and verbose = 1. */
 typedef struct {
   int32_t compute_cost;           /* 1 = build without -DNO_COST_CALCULATE */
   int32_t abort_on_cost_increase; /* XT_ICD_update.c:1185 */
@@ -39,6 +40,8 @@ typedef struct {
   int32_t write_files;            /* write the .bin outputs (default 1) */
   uint32_t order_seed;            /* seed of the per-cell voxel order */
   int32_t verbose;                /* [INFO] lines on the debug stream */
+  int32_t kernel;                 /* 0: pipelined wavefront kernel where it
+                                     applies, 1: always the one-voxel kernel */
 } tb_options;
 
 void tb_get_options(tb_options *opt);
@@ -130,7 +133,9 @@ int tb_stage_run(tb_stage *st, int32_t max_iter, double *costs,
  * events on the stage's stream: the voxel-update kernel time (ms), the
  * voxel updates performed (zero-skipped voxels excluded), and the
  * algorithmic bytes they moved (12 B per touched A-matrix entry: e read,
- * w read, e write).  Other kernels' time in other_ms. */
+ * w read, e write).  Other kernels' time in other_ms.  lds_launches counts
+ * the voxel-update launches that ran the LDS-resident kernel (the others ran
+ * the one-voxel-at-a-time kernel). */
 typedef struct {
   double voxel_ms;
   double other_ms;
@@ -139,6 +144,7 @@ typedef struct {
   int64_t zero_skips;
   int64_t entries;
   double alg_bytes;
+  int64_t lds_launches;
 } tb_stats;
 int tb_get_stats(tb_stage *st, tb_stats *out);
 int tb_reset_stats(tb_stage *st);

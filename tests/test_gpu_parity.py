@@ -34,6 +34,26 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
+@pytest.fixture(params=[0, 1], ids=["lds", "onevoxel"])
+def kernel(request):
+    """Voxel-update kernel: 0 = LDS-resident wavefront kernel where the cell
+    fits in LDS, 1 = the one-voxel-at-a-time kernel everywhere."""
+    old = tb.get_options().kernel
+    tb.set_options(kernel=request.param)
+    yield request.param
+    tb.set_options(kernel=old)
+
+
+def check_kernel(g, kernel):
+    """The requested kernel is the one that ran."""
+    st = g.stats()
+    assert st["voxel_launches"] > 0
+    if kernel == 0:
+        assert st["lds_launches"] == st["voxel_launches"], st
+    else:
+        assert st["lds_launches"] == 0, st
+
+
 def random_object(shape, mask, seed=1, scale=0.02):
     rng = np.random.default_rng(seed)
     return (rng.random(shape) * scale * mask[None, None]).astype(np.float32)
@@ -101,8 +121,8 @@ def test_cost_matches_oracle(quad):
         assert abs(a - b) <= 1e-11 * abs(b) + 1e-12
 
 
-def test_single_voxel_update():
-    log("single voxel")
+def test_single_voxel_update(kernel):
+    log(f"single voxel kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40)
     o, g = pair(scan)
     x = random_object(o.obj_shape, o.mask(), seed=3)
@@ -118,6 +138,7 @@ def test_single_voxel_update():
     e_o = o.err()
     e_g = g.error_sino()
     assert np.max(np.abs(e_g - e_o)) <= 2e-5 * max(1.0, np.abs(e_o).max())
+    check_kernel(g, kernel)
 
 
 @pytest.mark.parametrize("typ", [1, 2, 3])
@@ -176,10 +197,10 @@ def test_vsc_select_lists():
 
 
 @pytest.mark.parametrize("quad,rings", [(1, 0), (0, 2), (0, 1)])
-def test_icd_iterations_track_oracle(quad, rings):
+def test_icd_iterations_track_oracle(quad, rings, kernel):
     """Same colour schedule on both sides: the trajectories agree to FP32
     (non-stiff prior, see test_phases_track_oracle)."""
-    log(f"icd trajectory quad={quad} rings={rings}")
+    log(f"icd trajectory quad={quad} rings={rings} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, quad_convex=quad, rings=0.01 if rings else 0.0,
                       zingers=1e-3 if not quad else 0.0, remove_rings=rings, c_s=1.0, c_t=1.0)
     o, g = pair(scan, stage_idx=1 if rings else 0, prev_mult_z=1 if rings else 0)
@@ -195,12 +216,13 @@ def test_icd_iterations_track_oracle(quad, rings):
         log(f"  it {it}: rmse/range {rmse:.2e} cost {cg:.8e} vs {co:.8e} ratio {rg:.4f} vs {ro:.4f}")
         assert rmse < 1e-4
         assert abs(cg - co) <= 1e-5 * abs(co)
+    check_kernel(g, kernel)
 
 
-def test_converges_to_map():
+def test_converges_to_map(kernel):
     """GPU ICD reaches the scipy minimiser of the explicit MAP cost."""
     import mathref
-    log("converge to MAP")
+    log(f"converge to MAP kernel={kernel}")
     scan = scan_small(N=12, nz=4, n_views=16)
     o = pyorc.Problem(scan, opt=pyorc.options(), precision="f64")
     mp = mathref.MapProblem(o)
@@ -212,16 +234,17 @@ def test_converges_to_map():
     log(f"  gpu {costs[-1]:.10e} scipy {cmin:.10e} rmse {rmse:.2e}")
     assert abs(costs[-1] - cmin) <= 1e-5 * abs(cmin)
     assert rmse <= 5e-3
+    check_kernel(g, kernel)
 
 
-def test_phases_track_oracle(c=1.0):
+def test_phases_track_oracle(kernel, c=1.0):
     """Colour phases of iterations 1-2 one at a time (HICD, VSC, NHICD).
 
     With a stiff prior (c ~ 1e-2, sigma ~ 5e-3) early ICD trajectories are
     chaotic: two CPU oracles that differ only in float vs double arrays part
     by ~1% RMSE at iteration 2.  Trajectory parity is therefore checked with
     c = 1; stiff problems are checked at convergence (test_converges_to_map)."""
-    log(f"phase by phase c={c}")
+    log(f"phase by phase c={c} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, c_s=c, c_t=c)
     o, g = pair(scan)
     o.init_error_sino()
@@ -254,3 +277,34 @@ def test_phases_track_oracle(c=1.0):
                     mg, mo = g.magmap(), o.mag()
                     log(f"  mag diff max {np.max(np.abs(mg - mo)):.3e}")
                 assert err <= 1e-4, (it, j, name, err)
+    check_kernel(g, kernel)
+
+
+@pytest.mark.parametrize("mxy,mz,n_views,n_frames", [
+    (1, 1, 150, 1),   # 150 views in one frame: three views per lane
+    (1, 1, 200, 2),   # 100 views per frame: two per lane
+    (2, 2, 40, 2),    # second-finest stage: 5 bins x 2 rows per view
+    (2, 2, 200, 2),
+])
+def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
+    """Every instantiation of the voxel-update kernels follows the oracle."""
+    log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
+    scan = scan_small(N=32, nz=4, n_views=n_views, n_frames=n_frames, c_s=1.0, c_t=1.0)
+    stage = 1 if mxy > 1 else 0
+    o, g = pair(scan, stage_idx=stage, mult_xy=mxy, mult_z=mz,
+                prev_mult_z=2 * mz if stage else 0)
+    x = random_object(o.obj_shape, o.mask(), seed=5)
+    o.obj()[...] = x
+    g.set_object(x)
+    o.init_error_sino()
+    g.init_error_sino()
+    for it in range(1, 3):
+        o.icd_iteration(it)
+        g.icd_iteration(it)
+        xo, xg = o.obj(), g.object()
+        rmse = np.sqrt(np.mean((xg - xo) ** 2)) / (xo.max() - xo.min())
+        co, cg = o.cost()[0], g.cost()[0]
+        log(f"  it {it}: rmse/range {rmse:.2e} cost {cg:.8e} vs {co:.8e}")
+        assert rmse < 1e-4
+        assert abs(cg - co) <= 1e-5 * abs(co)
+    check_kernel(g, kernel)

@@ -31,13 +31,15 @@ EXPORTS = [
 class TbOptions(C.Structure):
     _fields_ = [("compute_cost", C.c_int32), ("abort_on_cost_increase", C.c_int32),
                 ("max_iterations", C.c_int32), ("no_nhicd", C.c_int32),
-                ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32)]
+                ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32),
+                ("kernel", C.c_int32)]
 
 
 class TbStats(C.Structure):
     _fields_ = [("voxel_ms", C.c_double), ("other_ms", C.c_double),
                 ("voxel_launches", C.c_int64), ("updates", C.c_int64),
-                ("zero_skips", C.c_int64), ("entries", C.c_int64), ("alg_bytes", C.c_double)]
+                ("zero_skips", C.c_int64), ("entries", C.c_int64), ("alg_bytes", C.c_double),
+                ("lds_launches", C.c_int64)]
 
 
 class TimbirError(RuntimeError):

@@ -34,6 +34,12 @@ constexpr double kPi4 = 0.785398163397448309615660845819875721;
 struct KP {
   int N_p, N_r, N_t, N_x, N_y, N_z, N_time, mz, K;
   double R0, RMax, delta_r, x0, y0, delta_xy, OffsetR;
+  // detector coordinate in bin units u = (r - R0) / delta_r: 1/delta_r,
+  // R0/delta_r, delta_xy/delta_r, (RMax - R0)/delta_r; delta_r/OffsetR
+  double u_inv, u_r0, u_dxy, u_max;
+  double u_kk;
+  int mxy;           // delta_xy / delta_r (the stage's x-y multiplier)
+  float T2h, T2l;    // T^2 as an unevaluated float sum (exact)
   float sSq, sSqp, sTq, sTqp, c_s, c_t, tfilt, alpha;
   float sfilt[27];
   float T, Td;       // generalized Huber threshold and T*delta
@@ -44,12 +50,26 @@ struct KP {
   const double *cosv, *sinv;
   const int *view_off, *view_idx;
   const uint8_t *mask;
-  int *nh_list, *nh_count;
+  int *nh_list, *nh_count;  // NHICD lines, raster order (all selected)
   int nh_cap;
+  int *nh_ord, *nh_ord_count;  // the same lines inside the mask, (y << 16) | x
+  int *ord, *ord_off;          // per-iteration voxel order ((y << 16) | x), chunk starts
+  int ord_cap;
+  float *dummy;                // per-lane scratch word for masked-off stores
+  const int *single;           // one-pixel sequence (single voxel update)
+  int nobj;                    // object words; obj and mag hold 64 more
   double *cell_voxmag;
   unsigned long long *cell_upd, *cell_zero, *cell_ent;
   uint32_t order_seed;
 };
+
+// LDS bytes of the LDS-resident voxel-update kernel for a cell of nv views:
+// the (e, w) block with an odd row stride, one dummy column window per lane,
+// then the views' detector-response rows.
+inline size_t icd_lds_bytes(int nv, int N_r, int mz, int nb) {
+  const size_t ew = (size_t)nv * (N_r * mz + 1) + (size_t)64 * nb * mz;
+  return ew * 2 * sizeof(float) + (size_t)nv * kHrStride * sizeof(float);
+}
 
 // Colour-schedule voxel order: keyed 4-round Feistel bijection on
 // [0, 2^(2h)) with cycle walking onto [0, N).  No reference counterpart

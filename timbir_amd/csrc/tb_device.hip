@@ -20,6 +20,8 @@ __global__ void k_to_ref_layout(KP, const float *, float *);
 __global__ void k_forward(KP, const int *, float *, int);
 template <int NT>
 __global__ void k_update_voxels(KP, const int2 *, int, int, int, int, int);
+template <int VPL, int NB, int MZ, int G>
+__global__ void k_icd_lds(KP, const int2 *, int, int, int, int);
 __global__ void k_offset_partial(KP, int, int, double *, double *);
 __global__ void k_offset_finalize(KP, int, const double *, const double *, double *, double *);
 __global__ void k_offset_gamma(KP, const double *, const double *, double *);
@@ -36,6 +38,7 @@ __global__ void k_mag_sum(KP, double *);
 __global__ void k_fill_f64(double *, size_t, double);
 __global__ void k_fill_f32(float *, size_t, float);
 __global__ void k_fill_u64(unsigned long long *, size_t);
+__global__ void k_build_order(KP, int);
 }  // namespace tb
 
 using namespace tb;
@@ -44,7 +47,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -89,6 +92,11 @@ struct tb_stage {
   uint8_t *mask = nullptr;
   int *nh_list = nullptr, *nh_count = nullptr;
   int nh_cap = 0;
+  int *nh_ord = nullptr, *nh_ord_count = nullptr, *ord = nullptr, *ord_off = nullptr;
+  int ord_cap = 0, ord_iter = -1;
+  float *dummy = nullptr;
+  int *single = nullptr;
+  size_t lds_max = 65536;
   int2 *cells[2] = {nullptr, nullptr};
   int ncells[2] = {0, 0};
   double *cell_voxmag = nullptr;
@@ -102,7 +110,7 @@ struct tb_stage {
   std::vector<hipEvent_t> ev;
   int ev_used = 0;
   double voxel_ms = 0, other_ms = 0;
-  long long voxel_launches = 0;
+  long long voxel_launches = 0, lds_launches = 0;
   hipEvent_t o0 = nullptr, o1 = nullptr;
 
   KP kp() const {
@@ -119,6 +127,15 @@ struct tb_stage {
     P.e = e; P.w = w; P.y = y; P.d = d; P.obj = obj; P.mag = mag; P.hr = hr;
     P.cosv = cosv; P.sinv = sinv; P.view_off = view_off_d; P.view_idx = view_idx_d;
     P.mask = mask; P.nh_list = nh_list; P.nh_count = nh_count; P.nh_cap = nh_cap;
+    P.nh_ord = nh_ord; P.nh_ord_count = nh_ord_count; P.ord = ord; P.ord_off = ord_off;
+    P.ord_cap = ord_cap; P.dummy = dummy; P.single = single; P.nobj = (int)nobj();
+    P.u_inv = 1.0 / delta_r; P.u_r0 = R0 / delta_r; P.u_dxy = delta_xy / delta_r;
+    P.u_max = (RMax - R0) / delta_r; P.u_kk = delta_r / OffsetR; P.mxy = mult_xy;
+    {
+      const double t2 = T * T;  // T is a float parameter: T^2 is exact in double
+      P.T2h = (float)t2;
+      P.T2l = (float)(t2 - (double)P.T2h);
+    }
     P.cell_voxmag = cell_voxmag; P.cell_upd = cell_upd; P.cell_zero = cell_zero;
     P.cell_ent = cell_ent; P.order_seed = g_opt.order_seed;
     return P;
@@ -185,7 +202,8 @@ extern "C" void tb_stage_destroy(tb_stage *st) {
   if (st->stream) (void)hipStreamSynchronize(st->stream);
   void *bufs[] = {st->e, st->w, st->y, st->d, st->obj, st->mag, st->hr, st->filt, st->cosv,
                   st->sinv, st->sfilt_d, st->view_off_d, st->view_idx_d, st->view_frame_d,
-                  st->mask, st->nh_list, st->nh_count, st->cells[0], st->cells[1],
+                  st->mask, st->nh_list, st->nh_count, st->nh_ord, st->nh_ord_count, st->ord,
+                  st->ord_off, st->dummy, st->single, st->cells[0], st->cells[1],
                   st->cell_voxmag, st->cell_upd, st->cell_zero, st->cell_ent, st->off_pnum,
                   st->off_pden, st->off_num, st->off_den, st->off_gamma, st->red};
   for (void *b : bufs)
@@ -273,6 +291,11 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
   st->updateVar = stage_idx > 0 && quad_convex == 0;
   st->var_est = var_est;
   st->N_x = st->N_r / mult_xy; st->N_y = st->N_r / mult_xy; st->N_z = st->N_t / mult_z;
+  // kernels index the object and a sinogram with 32-bit words and pack a
+  // pixel as (y << 16) | x
+  if (st->N_x >= 32768 || st->nobj() + 128 > (size_t)INT32_MAX ||
+      st->nsino() > (size_t)INT32_MAX)
+    return fail("tb_stage_create: problem too large for 32-bit indexing");
   st->delta_xy = mult_xy * st->delta_r;
   st->delta_z = mult_z * st->delta_t;
   st->OffsetR = (st->delta_xy / sqrt(2.0) + st->delta_r / 2.0) / kRespBins;
@@ -323,8 +346,10 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
   HIPCHK(dalloc(&st->w, ns));
   HIPCHK(dalloc(&st->y, ns));
   HIPCHK(dalloc(&st->d, (size_t)st->N_r * st->N_t));
-  HIPCHK(dalloc(&st->obj, no));
-  HIPCHK(dalloc(&st->mag, no));
+  // past the object: 64 words that take the stores of lanes with no voxel,
+  // then a NaN word that stands for a neighbour outside the object
+  HIPCHK(dalloc(&st->obj, no + 128));
+  HIPCHK(dalloc(&st->mag, no + 64));
   HIPCHK(dalloc(&st->filt, no));
   HIPCHK(dalloc(&st->hr, hr_h.size()));
   HIPCHK(dalloc(&st->cosv, st->N_p));
@@ -337,6 +362,19 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
   st->nh_cap = std::max(1, st->U / kNhicdK);
   HIPCHK(dalloc(&st->nh_list, (size_t)ncell * st->nh_cap));
   HIPCHK(dalloc(&st->nh_count, ncell));
+  HIPCHK(dalloc(&st->nh_ord, (size_t)ncell * st->nh_cap));
+  HIPCHK(dalloc(&st->nh_ord_count, ncell));
+  st->ord_cap = std::max(1, st->U);
+  HIPCHK(dalloc(&st->ord, (size_t)ncell * st->ord_cap));
+  HIPCHK(dalloc(&st->ord_off, (size_t)ncell * (kNhicdK + 1)));
+  HIPCHK(dalloc(&st->dummy, (size_t)ncell * 64));
+  HIPCHK(dalloc(&st->single, 64));
+  {
+    int dev = 0, lds = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    st->lds_max = (size_t)lds;
+  }
   HIPCHK(dalloc(&st->cell_voxmag, ncell));
   HIPCHK(dalloc(&st->cell_upd, ncell));
   HIPCHK(dalloc(&st->cell_zero, ncell));
@@ -392,6 +430,8 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(no)), dim3(256), 0, st->stream, st->obj, no, 0.f);
   hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(no)), dim3(256), 0, st->stream, st->mag, no, 0.f);
+  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(128), 0, st->stream, st->obj + no, (size_t)128,
+                     __builtin_nanf(""));
   hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(npix)), dim3(256), 0, st->stream, st->d,
                      (size_t)npix, 0.f);
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(ncell)), dim3(256), 0, st->stream,
@@ -399,6 +439,8 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
   // no NHICD lines until the first selection (all-zero bits = int 0)
   hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(ncell)), dim3(256), 0, st->stream,
                      reinterpret_cast<float *>(st->nh_count), (size_t)ncell, 0.f);
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(ncell)), dim3(256), 0, st->stream,
+                     reinterpret_cast<float *>(st->nh_ord_count), (size_t)ncell, 0.f);
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_upd,
                      (size_t)ncell);
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ncell)), dim3(256), 0, st->stream, st->cell_zero,
@@ -714,6 +756,17 @@ extern "C" int tb_variance_update(tb_stage *st) {
   return 0;
 }
 
+// the HICD voxel order of iteration `iter` for every cell, built once per
+// iteration
+static int ensure_order(tb_stage *st, int iter) {
+  if (st->ord_iter == iter) return 0;
+  KP P = st->kp();
+  hipLaunchKernelGGL(k_build_order, dim3(st->N_time * st->N_z), dim3(256), 0, st->stream, P, iter);
+  HIPCHK(hipGetLastError());
+  st->ord_iter = iter;
+  return 0;
+}
+
 static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk,
                          int seq, int pix, int count_mag) {
   if (ncells == 0) return 0;
@@ -724,7 +777,36 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     evp = ev_pair(st);
   }
   HIPCHK(hipEventRecord(evp[0], st->stream));
-  if (st->max_nv <= 64)
+  // LDS-resident wavefront kernel when a cell's (e, w) block and response
+  // rows fit in one workgroup's LDS and a view's column has at most 3 bins x
+  // 1 row (finest stage) or 5 bins x 2 rows
+  using lds_fn = void (*)(KP, const int2 *, int, int, int, int);
+  lds_fn fast = nullptr;
+  int nb = 0;
+  const int vpl = (st->max_nv + 63) / 64;
+  if (g_opt.kernel == 0) {
+    if (2 * st->mult_xy + 1 <= 3 && st->mult_z == 1) {
+      nb = 3;
+      if (vpl == 1) fast = k_icd_lds<1, 3, 1, 4>;
+      else if (vpl == 2) fast = k_icd_lds<2, 3, 1, 4>;
+      else if (vpl <= 4) fast = k_icd_lds<4, 3, 1, 2>;
+    } else if (2 * st->mult_xy + 1 <= 5 && st->mult_z == 2) {
+      nb = 5;
+      if (vpl == 1) fast = k_icd_lds<1, 5, 2, 4>;
+      else if (vpl == 2) fast = k_icd_lds<2, 5, 2, 2>;
+    }
+  }
+  const size_t lds = fast ? icd_lds_bytes(st->max_nv, st->N_r, st->mult_z, nb) : 0;
+  if (lds > st->lds_max || st->N_r < nb) fast = nullptr;
+  if (fast && seq == 0 && ensure_order(st, iter)) return -1;
+  if (fast) {
+    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fast),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fast, dim3(ncells), dim3(64), lds, st->stream, P, cells, iter, chunk, seq,
+                       count_mag);
+    st->lds_launches++;
+  }
+  else if (st->max_nv <= 64)
     hipLaunchKernelGGL(k_update_voxels<64>, dim3(ncells), dim3(64), 0, st->stream, P, cells,
                        iter, chunk, seq, pix, count_mag);
   else if (st->max_nv <= 128)
@@ -749,6 +831,8 @@ extern "C" int tb_voxel_update(tb_stage *st, int32_t f, int32_t z, int32_t y, in
   HIPCHK(dalloc(&dc, 1));
   HIPCHK(hipMemcpyAsync(dc, &c, sizeof(c), hipMemcpyHostToDevice, st->stream));
   const int pix = y * st->N_x + x;
+  const int packed = (y << 16) | x;
+  HIPCHK(hipMemcpyAsync(st->single, &packed, sizeof(int), hipMemcpyHostToDevice, st->stream));
   if (launch_update(st, dc, 1, 1, 0, 2, pix, 0)) return -1;
   HIPCHK(hipMemcpyAsync(value, st->obj + ((size_t)f * st->N_z + z) * st->nxy() + pix,
                         sizeof(float), hipMemcpyDeviceToHost, st->stream));
@@ -884,6 +968,7 @@ extern "C" int tb_get_stats(tb_stage *st, tb_stats *out) {
   out->voxel_ms = st->voxel_ms;
   out->other_ms = st->other_ms;
   out->voxel_launches = st->voxel_launches;
+  out->lds_launches = st->lds_launches;
   out->alg_bytes = 12.0 * (double)out->entries;
   return 0;
 }
@@ -902,6 +987,7 @@ extern "C" int tb_reset_stats(tb_stage *st) {
   st->voxel_ms = 0;
   st->other_ms = 0;
   st->voxel_launches = 0;
+  st->lds_launches = 0;
   return 0;
 }
 

@@ -1,0 +1,159 @@
+// Device helpers shared by the kernels of libtimbir_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "tb_common.h"
+
+namespace tb {
+
+__device__ __forceinline__ size_t sidx(const KP &P, int z, int v, int r, int m) {
+  return (((size_t)z * P.N_p + v) * P.N_r + r) * P.mz + m;
+}
+
+// A-matrix column of voxel (row, col) in one view: calcAMatrixColumnforAngle
+// (XT_AMatrix.c:50-110) in double with contraction off, so every bin and
+// value equals the reference's.  fn(bin, value) is called for each nonzero.
+template <class F>
+__device__ __forceinline__ void for_acol(const KP &P, const float *__restrict__ H, double c,
+                                         double s, int row, int col, F &&fn) {
+#pragma clang fp contract(off)
+  double x = P.x0 + ((double)col + 0.5) * P.delta_xy;
+  double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
+  double r = x * c - y * s;
+  double rmin = r - P.delta_xy, rmax = r + P.delta_xy;
+  if (rmax < P.R0 || rmin > P.RMax) return;
+  int lo = (int)floor((rmin - P.R0) / P.delta_r);
+  int hi = (int)floor((rmax - P.R0) / P.delta_r);
+  if (hi >= P.N_r) hi = P.N_r - 1;
+  if (lo < 0) lo = 0;
+  for (int j = lo; j <= hi; j++) {
+    double centre = P.R0 + ((double)j + 0.5) * P.delta_r;
+    double dist = fabs(r - centre);
+    int bin = (int)floor(dist / P.OffsetR);
+    if (bin >= 0 && bin < kRespBins) {
+      double w1 = dist - bin * P.OffsetR;
+      double w2 = P.OffsetR - w1;
+      double f1 = (w2 * (double)H[bin] + w1 * (double)H[bin + 1]) / P.OffsetR;
+      if (f1 > 0) fn(j, f1);
+    }
+  }
+}
+
+// Value of bin `j` of the column above, or 0 when j is outside it.
+__device__ __forceinline__ double acol_bin(const KP &P, const float *__restrict__ H, double c,
+                                           double s, int row, int col, int j) {
+#pragma clang fp contract(off)
+  double x = P.x0 + ((double)col + 0.5) * P.delta_xy;
+  double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
+  double r = x * c - y * s;
+  double rmin = r - P.delta_xy, rmax = r + P.delta_xy;
+  if (rmax < P.R0 || rmin > P.RMax) return 0.0;
+  int lo = (int)floor((rmin - P.R0) / P.delta_r);
+  int hi = (int)floor((rmax - P.R0) / P.delta_r);
+  if (hi >= P.N_r) hi = P.N_r - 1;
+  if (lo < 0) lo = 0;
+  if (j < lo || j > hi) return 0.0;
+  double centre = P.R0 + ((double)j + 0.5) * P.delta_r;
+  double dist = fabs(r - centre);
+  int bin = (int)floor(dist / P.OffsetR);
+  if (bin < 0 || bin >= kRespBins) return 0.0;
+  double w1 = dist - bin * P.OffsetR;
+  double w2 = P.OffsetR - w1;
+  double f1 = (w2 * (double)H[bin] + w1 * (double)H[bin + 1]) / P.OffsetR;
+  return f1 > 0 ? f1 : 0.0;
+}
+
+// ProjSelect b = |e sqrt(w)| < T evaluated as the reference does, in double
+// (XT_VoxUpdate.c:108-111); the float test decides all but a thin band.
+__device__ __forceinline__ bool selected(const KP &P, float e, float w) {
+  float u2 = e * e * w;
+  float T2 = P.T * P.T;
+  if (u2 < T2 * 0.99999f) return true;
+  if (u2 > T2 * 1.00001f) return false;
+  return fabs((double)e * sqrt((double)w)) < P.T_d;
+}
+
+// Neumaier compensated add.
+__device__ __forceinline__ void cadd(float &s, float &c, float x) {
+  float t = s + x;
+  if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
+  else c += (x - t) + s;
+  s = t;
+}
+
+// (s, c) pair sum over a wavefront with error-free TwoSum at each level.
+__device__ __forceinline__ void wave_csum(float &s, float &c) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    float so = __shfl_xor(s, off);
+    float co = __shfl_xor(c, off);
+    float t = s + so;
+    float bp = t - s;
+    float err = (s - (t - bp)) + (so - bp);
+    s = t;
+    c = c + co + err;
+  }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// qGGMRF rho'(d)/d for p = 1.2, q = 2 (XT_ICD_update.c:98-129): with q = 2
+// |d|^(q-2) = 1 and the d = 0 limit q/(sigma^q c) is the same expression.
+__device__ __forceinline__ float qggmrf_b(float d, float sq, float sqp, float c) {
+  float a = fabsf(d);
+  float t1 = powf(a, (float)(kMrfQ - kMrfP)) / sqp;
+  float t3 = c + t1;
+  return ((float)kMrfQ - (float)(kMrfQ - kMrfP) * t1 / t3) / (t3 * sq);
+}
+
+// Wavefront sums over DPP lane moves (quad_perm, row mirrors, and the gfx9
+// row broadcasts 15/31); the total lands in lane 63 and is read back into a
+// scalar register.  Every lane must be active.
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM = 0xF>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(dpp_i<CTRL, RM>(__float_as_int(v)));
+}
+template <int CTRL, int RM = 0xF>
+__device__ __forceinline__ double dpp_d(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_i<CTRL, RM>((int)(unsigned)b);
+  const unsigned hi = (unsigned)dpp_i<CTRL, RM>((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float wave_total(float v) {
+  v += dpp_f<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);       // row_half_mirror
+  v += dpp_f<0x140>(v);       // row_mirror: row sums in every lane of a row
+  v += dpp_f<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_f<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double wave_total_d(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  v += dpp_d<0x142, 0xA>(v);
+  v += dpp_d<0x143, 0xC>(v);
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+}  // namespace tb

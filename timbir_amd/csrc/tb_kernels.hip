@@ -9,118 +9,9 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "tb_common.h"
+#include "tb_devutil.h"
 
 namespace tb {
-
-__device__ __forceinline__ size_t sidx(const KP &P, int z, int v, int r, int m) {
-  return (((size_t)z * P.N_p + v) * P.N_r + r) * P.mz + m;
-}
-
-// A-matrix column of voxel (row, col) in one view: calcAMatrixColumnforAngle
-// (XT_AMatrix.c:50-110) in double with contraction off, so every bin and
-// value equals the reference's.  fn(bin, value) is called for each nonzero.
-template <class F>
-__device__ __forceinline__ void for_acol(const KP &P, const float *__restrict__ H, double c,
-                                         double s, int row, int col, F &&fn) {
-#pragma clang fp contract(off)
-  double x = P.x0 + ((double)col + 0.5) * P.delta_xy;
-  double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
-  double r = x * c - y * s;
-  double rmin = r - P.delta_xy, rmax = r + P.delta_xy;
-  if (rmax < P.R0 || rmin > P.RMax) return;
-  int lo = (int)floor((rmin - P.R0) / P.delta_r);
-  int hi = (int)floor((rmax - P.R0) / P.delta_r);
-  if (hi >= P.N_r) hi = P.N_r - 1;
-  if (lo < 0) lo = 0;
-  for (int j = lo; j <= hi; j++) {
-    double centre = P.R0 + ((double)j + 0.5) * P.delta_r;
-    double dist = fabs(r - centre);
-    int bin = (int)floor(dist / P.OffsetR);
-    if (bin >= 0 && bin < kRespBins) {
-      double w1 = dist - bin * P.OffsetR;
-      double w2 = P.OffsetR - w1;
-      double f1 = (w2 * (double)H[bin] + w1 * (double)H[bin + 1]) / P.OffsetR;
-      if (f1 > 0) fn(j, f1);
-    }
-  }
-}
-
-// Value of bin `j` of the column above, or 0 when j is outside it.
-__device__ __forceinline__ double acol_bin(const KP &P, const float *__restrict__ H, double c,
-                                           double s, int row, int col, int j) {
-#pragma clang fp contract(off)
-  double x = P.x0 + ((double)col + 0.5) * P.delta_xy;
-  double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
-  double r = x * c - y * s;
-  double rmin = r - P.delta_xy, rmax = r + P.delta_xy;
-  if (rmax < P.R0 || rmin > P.RMax) return 0.0;
-  int lo = (int)floor((rmin - P.R0) / P.delta_r);
-  int hi = (int)floor((rmax - P.R0) / P.delta_r);
-  if (hi >= P.N_r) hi = P.N_r - 1;
-  if (lo < 0) lo = 0;
-  if (j < lo || j > hi) return 0.0;
-  double centre = P.R0 + ((double)j + 0.5) * P.delta_r;
-  double dist = fabs(r - centre);
-  int bin = (int)floor(dist / P.OffsetR);
-  if (bin < 0 || bin >= kRespBins) return 0.0;
-  double w1 = dist - bin * P.OffsetR;
-  double w2 = P.OffsetR - w1;
-  double f1 = (w2 * (double)H[bin] + w1 * (double)H[bin + 1]) / P.OffsetR;
-  return f1 > 0 ? f1 : 0.0;
-}
-
-// ProjSelect b = |e sqrt(w)| < T evaluated as the reference does, in double
-// (XT_VoxUpdate.c:108-111); the float test decides all but a thin band.
-__device__ __forceinline__ bool selected(const KP &P, float e, float w) {
-  float u2 = e * e * w;
-  float T2 = P.T * P.T;
-  if (u2 < T2 * 0.99999f) return true;
-  if (u2 > T2 * 1.00001f) return false;
-  return fabs((double)e * sqrt((double)w)) < P.T_d;
-}
-
-// Neumaier compensated add.
-__device__ __forceinline__ void cadd(float &s, float &c, float x) {
-  float t = s + x;
-  if (fabsf(s) >= fabsf(x)) c += (s - t) + x;
-  else c += (x - t) + s;
-  s = t;
-}
-
-// (s, c) pair sum over a wavefront with error-free TwoSum at each level.
-__device__ __forceinline__ void wave_csum(float &s, float &c) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    float so = __shfl_xor(s, off);
-    float co = __shfl_xor(c, off);
-    float t = s + so;
-    float bp = t - s;
-    float err = (s - (t - bp)) + (so - bp);
-    s = t;
-    c = c + co + err;
-  }
-}
-
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-// qGGMRF rho'(d)/d for p = 1.2, q = 2 (XT_ICD_update.c:98-129): with q = 2
-// |d|^(q-2) = 1 and the d = 0 limit q/(sigma^q c) is the same expression.
-__device__ __forceinline__ float qggmrf_b(float d, float sq, float sqp, float c) {
-  float a = fabsf(d);
-  float t1 = powf(a, (float)(kMrfQ - kMrfP)) / sqp;
-  float t3 = c + t1;
-  return ((float)kMrfQ - (float)(kMrfQ - kMrfP) * t1 / t3) / (t3 * sq);
-}
 
 // ---------------------------------------------------------------------------
 // Input transpose: [view][row][col] -> [z][view][col][m], weights / var_est
@@ -701,7 +592,7 @@ __global__ __launch_bounds__(256) void k_vsc_select(KP P, const float *__restric
   // compaction in raster order: wave ballots + one cross-wave prefix
   __shared__ int s_wcnt[4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int base = 0;
+  int base = 0, base2 = 0;
   for (int start = 0; start < Nxy; start += blockDim.x) {
     const int i = start + threadIdx.x;
     const bool flag = (want > 0) && i < Nxy && __uint_as_float(vals[i]) > thresh;
@@ -717,8 +608,27 @@ __global__ __launch_bounds__(256) void k_vsc_select(KP P, const float *__restric
     }
     for (int w = 0; w < 4; w++) base += s_wcnt[w];
     __syncthreads();
+    // the same lines inside the mask, for the pipelined update; a selected
+    // line outside the mask only has its map entry cleared (XT_VoxUpdate.c:156)
+    const bool in = flag && P.mask[i] == 1;
+    if (flag && !in) P.mag[(size_t)cell * Nxy + i] = 0.0f;
+    const unsigned long long bal2 = __ballot(in);
+    const int inwave2 = __popcll(bal2 & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wcnt[wid] = __popcll(bal2);
+    __syncthreads();
+    int off2 = base2;
+    for (int w = 0; w < wid; w++) off2 += s_wcnt[w];
+    if (in) {
+      const int pos = off2 + inwave2;
+      if (pos < P.nh_cap) P.nh_ord[(size_t)cell * P.nh_cap + pos] = ((i / P.N_x) << 16) | (i % P.N_x);
+    }
+    for (int w = 0; w < 4; w++) base2 += s_wcnt[w];
+    __syncthreads();
   }
-  if (threadIdx.x == 0) P.nh_count[cell] = min(base, P.nh_cap);
+  if (threadIdx.x == 0) {
+    P.nh_count[cell] = min(base, P.nh_cap);
+    P.nh_ord_count[cell] = min(base2, P.nh_cap);
+  }
 }
 
 // sum of the magnitude map over masked pixels (convergence numerator,

@@ -1,0 +1,381 @@
+// LDS-resident ICD voxel update for gfx950: one wavefront per cell, lane = view.
+//
+// Every error-sinogram entry a cell (frame f, slice z) touches lies in the
+// block [views of f][N_r][rows of z], and no other cell of the same colour
+// touches it.  When that block fits in LDS (64 views x 256 bins x (e, w) is
+// 128 KB of the 160 KB), the workgroup copies it in, runs the cell's whole
+// voxel sequence against LDS, and copies e back at the end.  LDS operations of
+// one wavefront complete in order, so voxel k+1 reads exactly the e that
+// voxel k wrote: the sequential sweep of updateVoxels_Atten
+// (XT_VoxUpdate.c:117-281) is reproduced without atomics or bookkeeping.
+//
+// What stays in global memory is the object: the 28 neighbour values each
+// voxel needs.  They are loaded one group of G voxels ahead.  An in-plane
+// neighbour may be a voxel updated after its load was issued; every update
+// patches the in-flight copies of the voxels that have it as a neighbour.
+// The loop is laid out so that its one wait for global memory,
+// `s_waitcnt vmcnt(0)`, comes a whole group after the loads it covers:
+//   fill group g (column footprints, neighbour loads, next group's pixels)
+//   solve group g-1 from registers and LDS
+//   wait; store group g-1's voxel values; group g becomes current.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "tb_common.h"
+#include "tb_devutil.h"
+
+namespace tb {
+
+// Per-iteration voxel order of every cell: the keyed permutation of the
+// N_x*N_y pixels with unmasked pixels dropped, as (y << 16) | x, plus where
+// each of the K HICD chunks starts (chunks split the permutation index range,
+// as XT_ICD_update.c:983-986 splits the reference's list).  One workgroup per
+// cell.
+__global__ __launch_bounds__(256) void k_build_order(KP P, int iter) {
+  const int cell = blockIdx.x;
+  const int f = cell / P.N_z, z = cell - f * P.N_z;
+  const int Nxy = P.N_x * P.N_y;
+  const uint32_t key = perm_key(P.order_seed, iter, f, z);
+  int *ord = P.ord + (size_t)cell * P.ord_cap;
+  int *off = P.ord_off + (size_t)cell * (P.K + 1);
+  const int per = Nxy / P.K;
+  __shared__ int s_wcnt[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int base = 0;
+  for (int start = 0; start < Nxy; start += blockDim.x) {
+    const int n = start + threadIdx.x;
+    int pix = 0;
+    bool ok = false;
+    if (n < Nxy) {
+      pix = (int)perm((uint32_t)n, (uint32_t)Nxy, key);
+      ok = P.mask[pix] == 1;
+    }
+    const unsigned long long bal = __ballot(ok);
+    const int inwave = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wcnt[wid] = __popcll(bal);
+    __syncthreads();
+    int pos = base;
+    for (int w = 0; w < wid; w++) pos += s_wcnt[w];
+    pos += inwave;
+    if (ok && pos < P.ord_cap) ord[pos] = ((pix / P.N_x) << 16) | (pix % P.N_x);
+    // chunk j starts at permutation index j * per; the count of masked
+    // pixels before index n is pos (before this thread's own pixel)
+    if (n < Nxy && per > 0 && n % per == 0 && n / per < P.K) off[n / per] = pos;
+    for (int w = 0; w < 4; w++) base += s_wcnt[w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) off[P.K] = min(base, P.ord_cap);
+}
+
+__device__ __forceinline__ void wait_vm_all() {
+  __builtin_amdgcn_sched_barrier(0);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt untouched
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Generalized-Huber selection b = |e sqrt(w)| < T (XT_VoxUpdate.c:108-111)
+// decided exactly: e^2 w is carried as q + (error terms) with FMA residuals
+// and compared with T^2 = T2h + T2l, so the outcome is the exact one (the
+// reference's double evaluation differs from it only within 1e-16 of T).
+__device__ __forceinline__ bool huber_sel(const KP &P, float e, float w) {
+  const float p = e * e;
+  const float pe = fmaf(e, e, -p);
+  const float q = p * w;
+  const float qe = fmaf(p, w, -q);
+  const float corr = fmaf(pe, w, qe);
+  return (q - P.T2h) + (corr - P.T2l) < 0.0f;
+}
+
+constexpr int kNoPix = 0x7fffffff;  // empty slot (never equals a packed pixel)
+
+template <int VPL, int NB, int MZ, int G>
+__global__ __launch_bounds__(64) void k_icd_lds(KP P, const int2 *__restrict__ cells, int iter,
+                                                int chunk, int seq, int count_mag) {
+  extern __shared__ float2 s_ew[];  // [nv][RS] (e, w), lane windows; then H rows
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int lane = threadIdx.x;
+  const int Nxy = P.N_x * P.N_y;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+  const int RW = P.N_r * MZ;  // entries of one view
+  const int RS = RW + 1;      // LDS row stride, odd to spread banks
+  const int rows = nv * RS;
+  const int ldum = rows + lane * NB * MZ;  // the lane's own dummy window
+  float *s_h = reinterpret_cast<float *>(s_ew + rows + 64 * NB * MZ);
+
+  // the cell's sequence
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0) return;
+
+  // stage the cell's (e, w) block and the frame's detector-response rows
+  for (int q = 0; q < nv; q++) {
+    const size_t g = sidx(P, z, P.view_idx[vb + q], 0, 0);
+    for (int t = lane; t < RW; t += 64) s_ew[q * RS + t] = make_float2(P.e[g + t], P.w[g + t]);
+    if (lane == 0) s_ew[q * RS + RW] = make_float2(0.0f, 0.0f);
+  }
+  for (int t = rows + lane; t < rows + 64 * NB * MZ; t += 64) s_ew[t] = make_float2(0.0f, 0.0f);
+  for (int i = lane; i < nv * kHrStride; i += 64) {
+    const int q = i / kHrStride, b = i - q * kHrStride;
+    s_h[i] = P.hr[(size_t)P.view_idx[vb + q] * kHrStride + b];
+  }
+  __syncthreads();
+
+  // the lane's views: detector coordinate in bins u = xx*Au - yy*Bu + Cu
+  int qv[VPL];
+  bool vok[VPL];
+  double Au[VPL], Bu[VPL], Cu[VPL];
+#pragma unroll
+  for (int i = 0; i < VPL; i++) {
+    const int q = lane + 64 * i;
+    vok[i] = q < nv;
+    qv[i] = vok[i] ? q : 0;
+    const int v = P.view_idx[vb + qv[i]];
+    const double c = P.cosv[v], sn = P.sinv[v];
+    Au[i] = P.delta_xy * c * P.u_inv;
+    Bu[i] = P.delta_xy * sn * P.u_inv;
+    Cu[i] = ((P.x0 + 0.5 * P.delta_xy) * c - (P.y0 + 0.5 * P.delta_xy) * sn - P.R0) * P.u_inv;
+  }
+  const int m = P.mxy;
+  // the lane's neighbour (lanes 0..26 spatial [dz][dy][dx], 27/28 temporal)
+  int ndz = 0, ndy = 0, ndx = 0, ndt = 0;
+  float ng = 0.0f, nsq = 1.0f, nisqp = 1.0f, nc = 1.0f;
+  const bool nlane = lane < 29;
+  if (nlane) {
+    if (lane < 27) {
+      ndz = lane / 9 - 1;
+      ndy = (lane / 3) % 3 - 1;
+      ndx = lane % 3 - 1;
+      ng = P.sfilt[lane];
+      nsq = P.sSq, nisqp = 1.0f / P.sSqp, nc = P.c_s;
+    } else {
+      ndt = lane == 27 ? -1 : 1;
+      ng = P.tfilt;
+      nsq = P.sTq, nisqp = 1.0f / P.sTqp, nc = P.c_t;
+    }
+  }
+  const bool inplane = lane < 27 && ndz == 0 && lane != 13;
+  const int noff = ndy * 65536 + ndx;  // packed-coordinate offset of the neighbour
+  const int zz = z + ndz, ff = f + ndt;
+  const bool zok = nlane && zz >= 0 && zz < P.N_z && ff >= 0 && ff < P.N_time;
+  const int zc = min(max(zz, 0), P.N_z - 1), fc = min(max(ff, 0), P.N_time - 1);
+  const int nplane = (fc * P.N_z + zc) * Nxy;
+  const int cbase = cell * Nxy;
+  const float two_c = 2.0f * nc;
+
+  // current group (A) and the group in flight (B)
+  // Neighbour values that do not exist (outside the object, lanes 29..63)
+  // are NaN: their loads read the NaN word the object buffer carries past
+  // its end, and a patch not yet applied is NaN too.  Keeping validity in the
+  // values keeps per-lane booleans out of scalar registers.
+  int apix[G], abase[G][VPL], anid[G];
+  int bpix[G], bbase[G][VPL], bnid[G];
+  float aa[G][VPL][NB], ba[G][VPL][NB];
+  float anb[G], bnb[G], bnv[G];
+
+  auto fill = [&](int k, int pix) {
+    bpix[k] = pix;
+    const bool valid = pix != kNoPix;
+    const int yy = valid ? (pix >> 16) : 0, xx = valid ? (pix & 0xffff) : 0;
+    const double xd = (double)xx, yd = (double)yy;
+#pragma unroll
+    for (int i = 0; i < VPL; i++) {
+      // calcAMatrixColumnforAngle (XT_AMatrix.c:50-110) in bin units: bins
+      // floor(u) - m .. floor(u) + m, value H at |u - j - 0.5| delta_r/OffsetR
+      const double u = fma(xd, Au[i], fma(-yd, Bu[i], Cu[i]));
+      // bitwise tests throughout: short-circuit && would branch around the
+      // LDS reads below and serialise their latency
+      const bool inr = valid & vok[i] & (u + m >= 0.0) & (u - m <= P.u_max);
+      const int lo = (int)floor(u) - m;
+      const int loc = max(lo, 0), hic = min(lo + 2 * m, P.N_r - 1);
+      const int w0 = min(loc, P.N_r - NB);  // window of NB bins inside the row
+      bbase[k][i] = inr ? qv[i] * RS + w0 * MZ : ldum;
+      const double d = u - ((double)w0 + 0.5);
+      const float *H = s_h + qv[i] * kHrStride;
+#pragma unroll
+      for (int ub = 0; ub < NB; ub++) {
+        const double t = fabs(d - (double)ub) * P.u_kk;
+        const int bin = (int)t;
+        const bool ok = inr & (w0 + ub >= loc) & (w0 + ub <= hic) & (bin < kRespBins);
+        const int bb = ok ? bin : 0;
+        const float fr = (float)(t - (double)bin);
+        const float h0 = H[bb], h1 = H[bb + 1];
+        // a multiply rather than a select, so the reads are not moved under a
+        // branch
+        ba[k][i][ub] = fmaf(fr, h1 - h0, h0) * (ok ? 1.0f : 0.0f);
+      }
+    }
+    // neighbour value (the NaN word when it does not exist)
+    const int jj = yy + ndy, kk = xx + ndx;
+    const bool nok = valid & zok & (jj >= 0) & (jj < P.N_y) & (kk >= 0) & (kk < P.N_x);
+    bnb[k] = P.obj[(unsigned)(nok ? nplane + jj * P.N_x + kk : P.nobj + 64)];
+    bnid[k] = (valid & inplane) ? pix + noff : -2;
+    bnv[k] = __int_as_float(0x7fc00000);
+  };
+
+  double voxmag = 0.0;
+  unsigned long long nupd = 0, nzero = 0;
+  int nent = 0;
+
+  const int ngroups = (cnt + G - 1) / G;
+  // pixels of group 0
+  int raw = list[n0 + min(lane, cnt - 1)];
+  wait_vm_all();
+  int pnext[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) pnext[k] = k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    apix[k] = kNoPix;
+    anid[k] = -2;
+    anb[k] = __int_as_float(0x7fc00000);
+#pragma unroll
+    for (int i = 0; i < VPL; i++) {
+      abase[k][i] = ldum;
+#pragma unroll
+      for (int ub = 0; ub < NB; ub++) aa[k][i][ub] = 0.0f;
+    }
+  }
+
+  for (int g = 0; g <= ngroups; g++) {
+    // group g in flight
+#pragma unroll
+    for (int k = 0; k < G; k++) fill(k, pnext[k]);
+    raw = list[n0 + min((g + 1) * G + lane, cnt - 1)];
+
+    // solve group g - 1 (empty slots run as no-ops: no bins, dx = 0)
+    float rx = 0.0f, rm = 0.0f;
+    int ridx = P.nobj + lane;
+#pragma unroll
+    for (int s = 0; s < G; s++) {
+      const int pix = apix[s];
+      const bool valid = pix != kNoPix;
+      // theta1 / theta2 (XT_VoxUpdate.c:54-83) from the LDS block
+      float2 ew[VPL][NB][MZ];
+#pragma unroll
+      for (int i = 0; i < VPL; i++)
+#pragma unroll
+        for (int ub = 0; ub < NB; ub++)
+#pragma unroll
+          for (int mm = 0; mm < MZ; mm++) ew[i][ub][mm] = s_ew[abase[s][i] + ub * MZ + mm];
+      float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+      for (int i = 0; i < VPL; i++)
+#pragma unroll
+        for (int ub = 0; ub < NB; ub++) {
+          const float a = aa[s][i][ub];
+#pragma unroll
+          for (int mm = 0; mm < MZ; mm++) {
+            const float e = ew[i][ub][mm].x, w = ew[i][ub][mm].y;
+            const float cw_out =
+                P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
+            const float cw = huber_sel(P, e, w) ? w : cw_out;
+            const float ac = a * cw;  // cw is finite: e = 0 is always selected
+            s2 = fmaf(a, ac, s2);
+            s1 = fmaf(-e, ac, s1);
+            nent += a > 0.0f ? 1 : 0;
+          }
+        }
+      // prior surrogate (XT_ICD_update.c:134-183): qGGMRF b(d) for p = 1.2,
+      // q = 2, (2 - 0.8 t1/t3) / (t3 sigma^q) = (2c + 1.2 t1) / (t3^2 sigma^q)
+      const float nbv = anb[s];
+      const bool vld = (nbv == nbv) & (lane != 13);
+      const float V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
+      const bool anypos = __ballot((lane != 13) & (nbv > 0.0f)) != 0ull;
+      const float dlt = vld ? V - nbv : 0.0f;
+      const float t1 = __builtin_amdgcn_exp2f(0.8f * __builtin_amdgcn_logf(fabsf(dlt))) * nisqp;
+      const float t3 = nc + t1;
+      const float b = fmaf(1.2f, t1, two_c) * __builtin_amdgcn_rcpf(t3 * t3 * nsq);
+      const float gb = (vld ? ng : 0.0f) * b;
+      // sums over the wavefront: theta1 in double (it cancels), the rest in float
+      const float pb = wave_total(gb);
+      const float pbx = wave_total(-gb * dlt);
+      const float th1 = (float)wave_total_d((double)s1);
+      const float th2 = wave_total(s2);
+      // zero skipping (XT_VoxUpdate.c:236-259)
+      const bool skip = (V == 0.0f) && iter > 1 && !anypos;
+      const float step = P.alpha * (pbx - th1) * __builtin_amdgcn_rcpf(pb + th2);
+      const float xn = (skip || !valid) ? V : V + step;
+      const float dx = xn - V;
+      // e -= A dx in LDS (entries outside the column and empty slots write
+      // their unchanged value back)
+#pragma unroll
+      for (int i = 0; i < VPL; i++)
+#pragma unroll
+        for (int ub = 0; ub < NB; ub++)
+#pragma unroll
+          for (int mm = 0; mm < MZ; mm++)
+            s_ew[abase[s][i] + ub * MZ + mm].x = fmaf(-aa[s][i][ub], dx, ew[i][ub][mm].x);
+      // in-plane neighbour values of the voxels still to come
+#pragma unroll
+      for (int o = s + 1; o < G; o++) anb[o] = anid[o] == pix ? xn : anb[o];
+#pragma unroll
+      for (int o = 0; o < G; o++) bnv[o] = bnid[o] == pix ? xn : bnv[o];
+      // this voxel's value and map entry, stored by lane s after the wait
+      const int lin0 = cbase + (valid ? pix >> 16 : 0) * P.N_x + (valid ? pix & 0xffff : 0);
+      const int lin = valid ? lin0 : P.nobj + s;
+      rx = lane == s ? xn : rx;
+      rm = lane == s ? (skip ? 0.0f : fabsf(dx)) : rm;
+      ridx = lane == s ? lin : ridx;
+      voxmag += (valid && count_mag && !skip) ? fabs((double)xn) : 0.0;
+      nupd += (valid && !skip) ? 1 : 0;
+      nzero += (valid && skip) ? 1 : 0;
+    }
+
+    wait_vm_all();
+    P.obj[(unsigned)ridx] = rx;
+    P.mag[(unsigned)ridx] = rm;
+
+    // group g becomes current
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      apix[k] = bpix[k];
+      anid[k] = bnid[k];
+      anb[k] = bnv[k] == bnv[k] ? bnv[k] : bnb[k];
+#pragma unroll
+      for (int i = 0; i < VPL; i++) {
+        abase[k][i] = bbase[k][i];
+#pragma unroll
+        for (int ub = 0; ub < NB; ub++) aa[k][i][ub] = ba[k][i][ub];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < G; k++)
+      pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+  }
+
+  // e back to global memory
+  __syncthreads();
+  for (int q = 0; q < nv; q++) {
+    const size_t gq = sidx(P, z, P.view_idx[vb + q], 0, 0);
+    for (int t = lane; t < RW; t += 64) P.e[gq + t] = s_ew[q * RS + t].x;
+  }
+  const double ne = wave_sum_d((double)nent);
+  if (lane == 0 && seq != 2) {
+    P.cell_voxmag[cell] += voxmag;
+    P.cell_upd[cell] += nupd;
+    P.cell_zero[cell] += nzero;
+    P.cell_ent[cell] += (unsigned long long)ne;
+  }
+}
+
+// finest stage (m_xy = m_z = 1: at most 3 bins per view, one detector row)
+template __global__ void k_icd_lds<1, 3, 1, 4>(KP, const int2 *, int, int, int, int);
+template __global__ void k_icd_lds<2, 3, 1, 4>(KP, const int2 *, int, int, int, int);
+template __global__ void k_icd_lds<4, 3, 1, 2>(KP, const int2 *, int, int, int, int);
+// second finest (m_xy = m_z = 2: at most 5 bins, two rows)
+template __global__ void k_icd_lds<1, 5, 2, 4>(KP, const int2 *, int, int, int, int);
+template __global__ void k_icd_lds<2, 5, 2, 2>(KP, const int2 *, int, int, int, int);
+
+}  // namespace tb
