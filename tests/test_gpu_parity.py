@@ -34,10 +34,11 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1], ids=["lds", "onevoxel"])
+@pytest.fixture(params=[0, 1, 2], ids=["multiwave", "onevoxel", "onewave"])
 def kernel(request):
-    """Voxel-update kernel: 0 = LDS-resident wavefront kernel where the cell
-    fits in LDS, 1 = the one-voxel-at-a-time kernel everywhere."""
+    """Voxel-update kernel: 0 = LDS-resident multi-wavefront kernel where the
+    cell fits, 1 = the one-voxel-at-a-time kernel everywhere, 2 = the
+    LDS-resident one-wavefront kernel."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -48,7 +49,7 @@ def check_kernel(g, kernel):
     """The requested kernel is the one that ran."""
     st = g.stats()
     assert st["voxel_launches"] > 0
-    if kernel == 0:
+    if kernel in (0, 2):
         assert st["lds_launches"] == st["voxel_launches"], st
     else:
         assert st["lds_launches"] == 0, st

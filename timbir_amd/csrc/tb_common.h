@@ -71,6 +71,14 @@ inline size_t icd_lds_bytes(int nv, int N_r, int mz, int nb) {
   return ew * 2 * sizeof(float) + (size_t)nv * kHrStride * sizeof(float);
 }
 
+// Dynamic LDS bytes of the multi-wavefront kernel for a cell of nv views and
+// nth theta waves: the (e, w) block, one dummy word pair per theta-wave lane,
+// the views' detector-response rows.
+inline size_t icd_mw_lds_bytes(int nv, int N_r, int mz, int nth) {
+  const size_t ew = (size_t)nv * (N_r * mz + 1) + (size_t)nth * 64;
+  return ew * 2 * sizeof(float) + (size_t)nv * kHrStride * sizeof(float);
+}
+
 // Colour-schedule voxel order: keyed 4-round Feistel bijection on
 // [0, 2^(2h)) with cycle walking onto [0, N).  No reference counterpart
 // (the reference shuffles with random2, XT_ICD_update.c:500-532).

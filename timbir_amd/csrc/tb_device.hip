@@ -22,6 +22,8 @@ template <int NT>
 __global__ void k_update_voxels(KP, const int2 *, int, int, int, int, int);
 template <int VPL, int NB, int MZ, int G>
 __global__ void k_icd_lds(KP, const int2 *, int, int, int, int);
+template <int NB, int MZ, int MAXTH>
+__global__ void k_icd_mw(KP, const int2 *, int, int, int, int, int);
 __global__ void k_offset_partial(KP, int, int, double *, double *);
 __global__ void k_offset_finalize(KP, int, const double *, const double *, double *, double *);
 __global__ void k_offset_gamma(KP, const double *, const double *, double *);
@@ -777,36 +779,57 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     evp = ev_pair(st);
   }
   HIPCHK(hipEventRecord(evp[0], st->stream));
-  // LDS-resident wavefront kernel when a cell's (e, w) block and response
-  // rows fit in one workgroup's LDS and a view's column has at most 3 bins x
-  // 1 row (finest stage) or 5 bins x 2 rows
-  using lds_fn = void (*)(KP, const int2 *, int, int, int, int);
-  lds_fn fast = nullptr;
+  // kernel 0 (auto): the multi-wavefront LDS kernel, else the one-wavefront
+  // LDS kernel, else the one-voxel kernel; kernel 1: one-voxel only; kernel
+  // 2: one-wavefront LDS kernel where it applies.  The LDS kernels need the
+  // cell's (e, w) block in LDS and a view's column within 3 bins x 1 row
+  // (finest stage) or 5 bins x 2 rows.
   int nb = 0;
-  const int vpl = (st->max_nv + 63) / 64;
-  if (g_opt.kernel == 0) {
-    if (2 * st->mult_xy + 1 <= 3 && st->mult_z == 1) {
-      nb = 3;
-      if (vpl == 1) fast = k_icd_lds<1, 3, 1, 4>;
-      else if (vpl == 2) fast = k_icd_lds<2, 3, 1, 4>;
-      else if (vpl <= 4) fast = k_icd_lds<4, 3, 1, 2>;
-    } else if (2 * st->mult_xy + 1 <= 5 && st->mult_z == 2) {
-      nb = 5;
-      if (vpl == 1) fast = k_icd_lds<1, 5, 2, 4>;
-      else if (vpl == 2) fast = k_icd_lds<2, 5, 2, 2>;
+  if (2 * st->mult_xy + 1 <= 3 && st->mult_z == 1) nb = 3;
+  else if (2 * st->mult_xy + 1 <= 5 && st->mult_z == 2) nb = 5;
+  if (st->N_r < nb) nb = 0;
+  bool launched = false;
+  if (nb && g_opt.kernel == 0) {
+    using mw_fn = void (*)(KP, const int2 *, int, int, int, int, int);
+    const int epv = nb * st->mult_z;
+    const int nth = (st->max_nv + 64 / epv - 1) / (64 / epv);
+    const int vpw = (st->max_nv + nth - 1) / nth;
+    mw_fn fn = nullptr;
+    if (nb == 3) fn = nth <= 4 ? k_icd_mw<3, 1, 4> : nth <= 8 ? k_icd_mw<3, 1, 8>
+                    : nth <= 15 ? k_icd_mw<3, 1, 15> : nullptr;
+    else fn = nth <= 8 ? k_icd_mw<5, 2, 8> : nth <= 15 ? k_icd_mw<5, 2, 15> : nullptr;
+    const size_t lds = icd_mw_lds_bytes(st->max_nv, st->N_r, st->mult_z, nth);
+    if (fn && lds + 1024 <= st->lds_max) {
+      if (seq == 0 && ensure_order(st, iter)) return -1;
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
+                         chunk, seq, count_mag, vpw);
+      st->lds_launches++;
+      launched = true;
     }
   }
-  const size_t lds = fast ? icd_lds_bytes(st->max_nv, st->N_r, st->mult_z, nb) : 0;
-  if (lds > st->lds_max || st->N_r < nb) fast = nullptr;
-  if (fast && seq == 0 && ensure_order(st, iter)) return -1;
-  if (fast) {
-    HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fast),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(fast, dim3(ncells), dim3(64), lds, st->stream, P, cells, iter, chunk, seq,
-                       count_mag);
-    st->lds_launches++;
+  if (!launched && nb && (g_opt.kernel == 0 || g_opt.kernel == 2)) {
+    using lds_fn = void (*)(KP, const int2 *, int, int, int, int);
+    lds_fn fast = nullptr;
+    const int vpl = (st->max_nv + 63) / 64;
+    if (nb == 3) fast = vpl == 1 ? k_icd_lds<1, 3, 1, 4> : vpl == 2 ? k_icd_lds<2, 3, 1, 4>
+                      : vpl <= 4 ? k_icd_lds<4, 3, 1, 2> : nullptr;
+    else fast = vpl == 1 ? k_icd_lds<1, 5, 2, 4> : vpl == 2 ? k_icd_lds<2, 5, 2, 2> : nullptr;
+    const size_t lds = icd_lds_bytes(st->max_nv, st->N_r, st->mult_z, nb);
+    if (fast && lds <= st->lds_max) {
+      if (seq == 0 && ensure_order(st, iter)) return -1;
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fast),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(fast, dim3(ncells), dim3(64), lds, st->stream, P, cells, iter, chunk,
+                         seq, count_mag);
+      st->lds_launches++;
+      launched = true;
+    }
   }
-  else if (st->max_nv <= 64)
+  if (launched) {
+    // counted above
+  } else if (st->max_nv <= 64)
     hipLaunchKernelGGL(k_update_voxels<64>, dim3(ncells), dim3(64), 0, st->stream, P, cells,
                        iter, chunk, seq, pix, count_mag);
   else if (st->max_nv <= 128)

@@ -118,20 +118,37 @@ __device__ __forceinline__ float qggmrf_b(float d, float sq, float sqp, float c)
 
 // Wavefront sums over DPP lane moves (quad_perm, row mirrors, and the gfx9
 // row broadcasts 15/31); the total lands in lane 63 and is read back into a
-// scalar register.  Every lane must be active.
+// scalar register.  Every lane must be active.  Moves that write every lane
+// need no initial value; the two broadcasts write only rows 1/3 and 2/3, the
+// other rows keep the 0 they start from.
+template <int CTRL>
+__device__ __forceinline__ int dpp_all(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
 template <int CTRL, int RM>
-__device__ __forceinline__ int dpp_i(int v) {
+__device__ __forceinline__ int dpp_rows(int v) {
   return __builtin_amdgcn_update_dpp(0, v, CTRL, RM, 0xF, false);
 }
 template <int CTRL, int RM = 0xF>
 __device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float(dpp_i<CTRL, RM>(__float_as_int(v)));
+  return __int_as_float(RM == 0xF ? dpp_all<CTRL>(__float_as_int(v))
+                                  : dpp_rows<CTRL, RM>(__float_as_int(v)));
 }
 template <int CTRL, int RM = 0xF>
 __device__ __forceinline__ double dpp_d(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)dpp_i<CTRL, RM>((int)(unsigned)b);
-  const unsigned hi = (unsigned)dpp_i<CTRL, RM>((int)(unsigned)(b >> 32));
+  const int l = (int)(unsigned)b, h = (int)(unsigned)(b >> 32);
+  const unsigned lo = (unsigned)(RM == 0xF ? dpp_all<CTRL>(l) : dpp_rows<CTRL, RM>(l));
+  const unsigned hi = (unsigned)(RM == 0xF ? dpp_all<CTRL>(h) : dpp_rows<CTRL, RM>(h));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane63_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63_d(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 __device__ __forceinline__ float wave_total(float v) {
@@ -141,7 +158,7 @@ __device__ __forceinline__ float wave_total(float v) {
   v += dpp_f<0x140>(v);       // row_mirror: row sums in every lane of a row
   v += dpp_f<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
   v += dpp_f<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  return lane63_f(v);
 }
 __device__ __forceinline__ double wave_total_d(double v) {
   v += dpp_d<0xB1>(v);
@@ -150,10 +167,42 @@ __device__ __forceinline__ double wave_total_d(double v) {
   v += dpp_d<0x140>(v);
   v += dpp_d<0x142, 0xA>(v);
   v += dpp_d<0x143, 0xC>(v);
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  return lane63_d(v);
+}
+// A double and a float sum taken level by level, so that each level's lane
+// moves fill the other tree's wait states.
+__device__ __forceinline__ void wave_total_df(double &d, float &f) {
+  d += dpp_d<0xB1>(d);
+  f += dpp_f<0xB1>(f);
+  d += dpp_d<0x4E>(d);
+  f += dpp_f<0x4E>(f);
+  d += dpp_d<0x141>(d);
+  f += dpp_f<0x141>(f);
+  d += dpp_d<0x140>(d);
+  f += dpp_f<0x140>(f);
+  d += dpp_d<0x142, 0xA>(d);
+  f += dpp_f<0x142, 0xA>(f);
+  d += dpp_d<0x143, 0xC>(d);
+  f += dpp_f<0x143, 0xC>(f);
+  d = lane63_d(d);
+  f = lane63_f(f);
+}
+// Two float sums taken level by level.
+__device__ __forceinline__ void wave_total_ff(float &a, float &b) {
+  a += dpp_f<0xB1>(a);
+  b += dpp_f<0xB1>(b);
+  a += dpp_f<0x4E>(a);
+  b += dpp_f<0x4E>(b);
+  a += dpp_f<0x141>(a);
+  b += dpp_f<0x141>(b);
+  a += dpp_f<0x140>(a);
+  b += dpp_f<0x140>(b);
+  a += dpp_f<0x142, 0xA>(a);
+  b += dpp_f<0x142, 0xA>(b);
+  a += dpp_f<0x143, 0xC>(a);
+  b += dpp_f<0x143, 0xC>(b);
+  a = lane63_f(a);
+  b = lane63_f(b);
 }
 
 }  // namespace tb

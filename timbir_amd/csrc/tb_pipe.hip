@@ -307,7 +307,9 @@ __global__ __launch_bounds__(64) void k_icd_lds(KP P, const int2 *__restrict__ c
       const bool skip = (V == 0.0f) && iter > 1 && !anypos;
       const float step = P.alpha * (pbx - th1) * __builtin_amdgcn_rcpf(pb + th2);
       const float xn = (skip || !valid) ? V : V + step;
-      const float dx = xn - V;
+      // an empty slot's V is the NaN of a missing neighbour: its dx must be
+      // 0 so that the unchanged e values it writes back stay finite
+      const float dx = valid ? xn - V : 0.0f;
       // e -= A dx in LDS (entries outside the column and empty slots write
       // their unchanged value back)
 #pragma unroll
@@ -369,6 +371,339 @@ __global__ __launch_bounds__(64) void k_icd_lds(KP P, const int2 *__restrict__ c
     P.cell_ent[cell] += (unsigned long long)ne;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Multi-wavefront form: one workgroup per cell, split by role.
+//
+// Waves 0..nth-1 ("theta waves") each own vpw views of the cell, one A-matrix
+// entry (view, bin, row) per lane, and the LDS rows of those views: no other
+// wave reads or writes them, so the in-order LDS argument above still gives
+// the sequential sweep.  The last wave ("prior wave") owns the object side:
+// neighbour prefetch one group ahead (with patching, as above), the qGGMRF
+// prior, and the object / map stores.  Per voxel every wave publishes its
+// partial sums in LDS and meets the others at one barrier; then every wave
+// evaluates the same closed-form step from the same sums in the same order,
+// so all waves hold the identical dx.
+// ---------------------------------------------------------------------------
+struct StepIn {
+  double t1[2][16];  // per theta wave, double-buffered by voxel parity
+  float t2[2][16];
+  float pr[2][4];    // pb, pbx, V, skip
+  unsigned long long nent[16];
+};
+
+template <int MAXTH>
+__device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int par, int nth,
+                                            float *V_out, bool *skip_out) {
+#pragma clang fp contract(off)
+  // every slot is read at once (one LDS latency); slots of absent waves
+  // hold zero
+  double T1 = 0.0;
+  float T2 = 0.0f;
+#pragma unroll
+  for (int w = 0; w < MAXTH; w++) {
+    T1 += S.t1[par][w];
+    T2 += S.t2[par][w];
+  }
+  (void)nth;
+  const float pb = S.pr[par][0], pbx = S.pr[par][1], V = S.pr[par][2];
+  const bool skip = S.pr[par][3] != 0.0f;
+  const float th1 = (float)T1;
+  const float num = P.alpha * (pbx - th1);
+  const float step = num * __builtin_amdgcn_rcpf(pb + T2);
+  *V_out = V;
+  *skip_out = skip;
+  return skip ? V : V + step;
+}
+
+template <int NB, int MZ, int MAXTH>
+__global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ cells, int iter,
+                                                 int chunk, int seq, int count_mag, int vpw) {
+  constexpr int EPV = NB * MZ;  // entries of one view
+  constexpr int G = 8;          // prior wave: voxels per prefetch group
+  extern __shared__ float2 s_ew[];  // [nv][RS] (e, w), per-lane dummies, then H rows
+  __shared__ StepIn S;
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int Nxy = P.N_x * P.N_y;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+  const int RW = P.N_r * MZ;
+  const int RS = RW + 1;
+  const int rows = nv * RS;
+  float *s_h = reinterpret_cast<float *>(s_ew + rows + nth * 64);
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0) return;
+
+  // stage the (e, w) block (whole rows per wave) and the response rows
+  const int nwaves = nth + 1;
+  for (int q = wave; q < nv; q += nwaves) {
+    const size_t g = sidx(P, z, P.view_idx[vb + q], 0, 0);
+    for (int t = lane; t < RW; t += 64) s_ew[q * RS + t] = make_float2(P.e[g + t], P.w[g + t]);
+    if (lane == 0) s_ew[q * RS + RW] = make_float2(0.0f, 0.0f);
+  }
+  for (int t = threadIdx.x; t < nth * 64; t += blockDim.x) s_ew[rows + t] = make_float2(0.f, 0.f);
+  for (int i = threadIdx.x; i < nv * kHrStride; i += blockDim.x) {
+    const int q = i / kHrStride, b = i - q * kHrStride;
+    s_h[i] = P.hr[(size_t)P.view_idx[vb + q] * kHrStride + b];
+  }
+  if (threadIdx.x < 32) {
+    (&S.t1[0][0])[threadIdx.x] = 0.0;
+    (&S.t2[0][0])[threadIdx.x] = 0.0f;
+  }
+  __syncthreads();
+
+  if (wave < nth) {
+    // ---------------- theta wave ----------------
+    const int lq = lane / EPV, ent = lane - lq * EPV;
+    const int ub = ent / MZ, mm = ent - ub * MZ;
+    const int q = wave * vpw + lq;
+    const bool active = lq < vpw && q < nv;
+    const int qv = active ? q : 0;
+    const int v = P.view_idx[vb + qv];
+    const double c = P.cosv[v], sn = P.sinv[v];
+    const double Au = P.delta_xy * c * P.u_inv, Bu = P.delta_xy * sn * P.u_inv;
+    const double Cu =
+        ((P.x0 + 0.5 * P.delta_xy) * c - (P.y0 + 0.5 * P.delta_xy) * sn - P.R0) * P.u_inv;
+    const int m = P.mxy;
+    const int dummy = rows + wave * 64 + lane;
+    const float *H = s_h + qv * kHrStride;
+
+    // the entry of voxel `pix` this lane owns: value and LDS index
+    auto column = [&](int pix, float &a, int &at) {
+      const bool valid = pix != kNoPix;
+      const int yy = valid ? (pix >> 16) : 0, xx = valid ? (pix & 0xffff) : 0;
+      const double u = fma((double)xx, Au, fma(-(double)yy, Bu, Cu));
+      const bool inr = valid & active & (u + m >= 0.0) & (u - m <= P.u_max);
+      const int lo = (int)floor(u) - m;
+      const int loc = max(lo, 0), hic = min(lo + 2 * m, P.N_r - 1);
+      const int w0 = min(loc, P.N_r - NB);
+      const int j = w0 + ub;
+      const double t = fabs(u - ((double)j + 0.5)) * P.u_kk;
+      const int bin = (int)t;
+      const bool ok = inr & (j >= loc) & (j <= hic) & (bin < kRespBins);
+      const int bb = ok ? bin : 0;
+      const float fr = (float)(t - (double)bin);
+      const float h0 = H[bb], h1 = H[bb + 1];
+      a = fmaf(fr, h1 - h0, h0) * (ok ? 1.0f : 0.0f);
+      at = inr ? qv * RS + j * MZ + mm : dummy;
+    };
+
+    // list window: entries [wb, wb + 64) in w0r; the next 64 are in flight
+    // in w1r (loaded 64 voxels before they are needed)
+    int wb = 0;
+    int w0r = list[n0 + min(lane, cnt - 1)];
+    int w1r = list[n0 + min(64 + lane, cnt - 1)];
+    auto pix_at = [&](int n) -> int {
+      return n < cnt ? __builtin_amdgcn_readlane(w0r, n - wb) : kNoPix;
+    };
+
+    float a, an;
+    int at, atn;
+    column(pix_at(0), a, at);
+    unsigned long long nent = 0;
+    for (int n = 0; n < cnt; n++) {
+      const int par = n & 1;
+      const float2 ew = s_ew[at];
+      const float e = ew.x, w = ew.y;
+      const float cw_out = P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
+      const float cw = huber_sel(P, e, w) ? w : cw_out;
+      const float ac = a * cw;
+      double t1 = (double)(-e * ac);
+      float t2 = a * ac;
+      wave_total_df(t1, t2);
+      if (lane == 0) {
+        S.t1[par][wave] = t1;
+        S.t2[par][wave] = t2;
+      }
+      nent += a > 0.0f ? 1 : 0;
+      // next voxel's entry while the other waves finish
+      if (n + 1 - wb >= 64) {
+        wb += 64;
+        w0r = w1r;
+        w1r = list[n0 + min(wb + 64 + lane, cnt - 1)];
+      }
+      column(pix_at(n + 1), an, atn);
+      __syncthreads();
+      float V;
+      bool skip;
+      const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
+      const float dx = xn - V;
+      s_ew[at].x = fmaf(-a, dx, e);
+      a = an;
+      at = atn;
+    }
+    const double ne = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)ne;
+  } else {
+    // ---------------- prior wave ----------------
+    int ndz = 0, ndy = 0, ndx = 0, ndt = 0;
+    float ng = 0.0f, nsq = 1.0f, nisqp = 1.0f, nc = 1.0f;
+    const bool nlane = lane < 29;
+    if (nlane) {
+      if (lane < 27) {
+        ndz = lane / 9 - 1;
+        ndy = (lane / 3) % 3 - 1;
+        ndx = lane % 3 - 1;
+        ng = P.sfilt[lane];
+        nsq = P.sSq, nisqp = 1.0f / P.sSqp, nc = P.c_s;
+      } else {
+        ndt = lane == 27 ? -1 : 1;
+        ng = P.tfilt;
+        nsq = P.sTq, nisqp = 1.0f / P.sTqp, nc = P.c_t;
+      }
+    }
+    const bool inplane = lane < 27 && ndz == 0 && lane != 13;
+    const int noff = ndy * 65536 + ndx;
+    const int zz = z + ndz, ff = f + ndt;
+    const bool zok = nlane && zz >= 0 && zz < P.N_z && ff >= 0 && ff < P.N_time;
+    const int zc = min(max(zz, 0), P.N_z - 1), fc = min(max(ff, 0), P.N_time - 1);
+    const int nplane = (fc * P.N_z + zc) * Nxy;
+    const int cbase = cell * Nxy;
+    const float two_c = 2.0f * nc;
+
+    int apix[G], anid[G], bpix[G], bnid[G];
+    float anb[G], bnb[G], bnv[G];
+    // packed pixels of the pending voxels, one per lane (A in lanes 0..G-1,
+    // B in lanes G..2G-1): one vector test per update finds the rare case of
+    // an in-plane neighbour among them
+    int pend = kNoPix;
+    auto fill = [&](int k, int pix) {
+      bpix[k] = pix;
+      const bool valid = pix != kNoPix;
+      const int yy = valid ? (pix >> 16) : 0, xx = valid ? (pix & 0xffff) : 0;
+      const int jj = yy + ndy, kk = xx + ndx;
+      const bool nok = valid & zok & (jj >= 0) & (jj < P.N_y) & (kk >= 0) & (kk < P.N_x);
+      bnb[k] = P.obj[(unsigned)(nok ? nplane + jj * P.N_x + kk : P.nobj + 64)];
+      bnid[k] = (valid & inplane) ? pix + noff : -2;
+      bnv[k] = __int_as_float(0x7fc00000);
+    };
+    double voxmag = 0.0;
+    unsigned long long nupd = 0, nzero = 0;
+    const int ngroups = (cnt + G - 1) / G;
+    int raw = list[n0 + min(lane, cnt - 1)];
+    wait_vm_all();
+    int pnext[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) pnext[k] = k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      apix[k] = kNoPix;
+      anid[k] = -2;
+      anb[k] = __int_as_float(0x7fc00000);
+    }
+    for (int g = 0; g <= ngroups; g++) {
+#pragma unroll
+      for (int k = 0; k < G; k++) fill(k, pnext[k]);
+      raw = list[n0 + min((g + 1) * G + lane, cnt - 1)];
+#pragma unroll
+      for (int k = 0; k < G; k++) pend = lane == G + k ? bpix[k] : pend;
+      float rx = 0.0f, rm = 0.0f;
+      int ridx = P.nobj + lane;
+#pragma unroll
+      for (int s = 0; s < G; s++) {
+        const int pix = apix[s];
+        if (pix == kNoPix) continue;  // only past the end of the sequence
+        const int n = (g - 1) * G + s;
+        const int par = n & 1;
+        const float nbv = anb[s];
+        const bool vld = (nbv == nbv) & (lane != 13);
+        const float V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
+        const bool anypos = __ballot((lane != 13) & (nbv > 0.0f)) != 0ull;
+        const float dlt = vld ? V - nbv : 0.0f;
+        const float t1 =
+            __builtin_amdgcn_exp2f(0.8f * __builtin_amdgcn_logf(fabsf(dlt))) * nisqp;
+        const float t3 = nc + t1;
+        const float b = fmaf(1.2f, t1, two_c) * __builtin_amdgcn_rcpf(t3 * t3 * nsq);
+        const float gb = (vld ? ng : 0.0f) * b;
+        float pb = gb, pbx = -gb * dlt;
+        wave_total_ff(pb, pbx);
+        const bool skip = (V == 0.0f) && iter > 1 && !anypos;
+        if (lane == 0) {
+          S.pr[par][0] = pb;
+          S.pr[par][1] = pbx;
+          S.pr[par][2] = V;
+          S.pr[par][3] = skip ? 1.0f : 0.0f;
+        }
+        __syncthreads();
+        float Vs;
+        bool sk;
+        const float xn = solve_step<MAXTH>(P, S, par, nth, &Vs, &sk);
+        const float dx = xn - V;
+        // (dy + 1) * 65536 + (dx + 1) with |dy|, |dx| <= 1 for a neighbour
+        const unsigned dd = (unsigned)(pend - pix + 65537);
+        const bool adj = (pend != kNoPix) & (dd != 65537u) & ((dd & 0xffffu) <= 2u) &
+                         (dd <= 131074u);
+        if (__ballot(adj) != 0ull) {
+#pragma unroll
+          for (int o = s + 1; o < G; o++) anb[o] = anid[o] == pix ? xn : anb[o];
+#pragma unroll
+          for (int o = 0; o < G; o++) bnv[o] = bnid[o] == pix ? xn : bnv[o];
+        }
+        const int lin = cbase + (pix >> 16) * P.N_x + (pix & 0xffff);
+        rx = lane == s ? xn : rx;
+        rm = lane == s ? (skip ? 0.0f : fabsf(dx)) : rm;
+        ridx = lane == s ? lin : ridx;
+        voxmag += (count_mag && !skip) ? fabs((double)xn) : 0.0;
+        nupd += skip ? 0 : 1;
+        nzero += skip ? 1 : 0;
+      }
+      wait_vm_all();
+      P.obj[(unsigned)ridx] = rx;
+      P.mag[(unsigned)ridx] = rm;
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        apix[k] = bpix[k];
+        anid[k] = bnid[k];
+        anb[k] = bnv[k] == bnv[k] ? bnv[k] : bnb[k];
+      }
+      // B moves to lanes 0..G-1
+      pend = __shfl(pend, (lane + G) & 63);
+      pend = lane < G ? pend : kNoPix;
+#pragma unroll
+      for (int k = 0; k < G; k++)
+        pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+    }
+    if (lane == 0 && seq != 2) {
+      P.cell_voxmag[cell] += voxmag;
+      P.cell_upd[cell] += nupd;
+      P.cell_zero[cell] += nzero;
+    }
+  }
+
+  // e back to global memory; entry count
+  __syncthreads();
+  for (int q = wave; q < nv; q += nwaves) {
+    const size_t gq = sidx(P, z, P.view_idx[vb + q], 0, 0);
+    for (int t = lane; t < RW; t += 64) P.e[gq + t] = s_ew[q * RS + t].x;
+  }
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    P.cell_ent[cell] += ne;
+  }
+}
+
+template __global__ void k_icd_mw<3, 1, 4>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<3, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<3, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<5, 2, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<5, 2, 15>(KP, const int2 *, int, int, int, int, int);
 
 // finest stage (m_xy = m_z = 1: at most 3 bins per view, one detector row)
 template __global__ void k_icd_lds<1, 3, 1, 4>(KP, const int2 *, int, int, int, int);
