@@ -30,6 +30,23 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "time-to-converged MAP (1e-4 rel) & voxel updates/s, learned vs plain init"
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
+                               "pmc_traffic_issue_k_icd_mw_interlaced4d_256x256x8x16.md")
+
+
+def measured_traffic():
+    """HBM bytes per voxel-kernel launch of the full workload, from the
+    committed FETCH_SIZE / WRITE_SIZE counter summary (a counter run cannot
+    happen inside the timed run itself)."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -64,6 +81,67 @@ def cpu_baseline(scan, stage_kw, threads: int, budget_s: float = 20.0):
                        f"{upd} voxel updates in {el:.1f} s")
 
 
+def multires_schedule(proj_rows: int, proj_cols: int):
+    """(mult_xy, mult_z) per stage, coarse to fine, as reconstruct() picks
+    them (XT_Engine.c:111-141 with one node)."""
+    import math
+    nres = int(math.log(proj_cols / 64.0) / math.log(2.0) + 1)
+    nres = max(2, min(8, nres))
+    mxy, mz = [1], [1]
+    for i in range(1, nres):
+        if proj_cols % (mxy[-1] * 2) != 0:
+            nres = i
+            break
+        mxy.append(mxy[-1] * 2)
+        mz.append(mz[-1] * 2 if proj_rows % (mz[-1] * 2) == 0 and proj_rows // (mz[-1] * 2) >= 2
+                  else mz[-1])
+    return list(zip(mxy[::-1], mz[::-1]))
+
+
+def time_to_converged(scan, finest_cap: int = 400, tol: float = 1e-4):
+    """Plain (zero) initialisation, reconstruct()'s multires schedule on the
+    GPU: coarse stages run to the reference's stopping rule, then the finest
+    stage runs on with the cost evaluated every iteration until it settles
+    (relative change < 1e-9, or finest_cap iterations).  C* is that settled
+    cost; the result is the wall time at which the finest stage's cost first
+    came within tol * |C*| of it, counted from the start of the first stage."""
+    import timbir_amd as tb
+    sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
+    t0 = time.perf_counter()
+    prev = None
+    var_est = 1.0
+    trace = []
+    for s, (mxy, mz) in enumerate(sched):
+        st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
+                      prev_mult_z=sched[s - 1][1] if s else 0, var_est=var_est)
+        if prev is not None:
+            st.init_from_coarse(prev["obj"], prev["nz"], prev["mag"])
+            st.set_offset(prev["off"])
+        st.init_error_sino()
+        finest = s == len(sched) - 1
+        last = None
+        for it in range(1, (finest_cap if finest else 1000) + 1):
+            conv, _ = st.icd_iteration(it)
+            if finest:
+                c = st.cost()[0]
+                st.synchronize()
+                trace.append((time.perf_counter() - t0, c))
+                if last is not None and abs(c - last) <= 1e-9 * abs(c):
+                    break
+                last = c
+            elif conv and it > 1:
+                break
+        if not finest:
+            st.synchronize()
+            prev = dict(obj=st.object(), nz=st.info["N_z"], mag=st.magmap(), off=st.offset())
+            var_est = st.var_est()
+        st.close()
+    cstar = trace[-1][1]
+    hit = next((t for t, c in trace if abs(c - cstar) <= tol * abs(cstar)), None)
+    return dict(seconds=hit, tol=tol, c_star=cstar, finest_iterations=len(trace),
+                schedule=[list(x) for x in sched], init="zero (plain)")
+
+
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
     """A plain device-to-device copy of nbytes, reps times: a counter run of
     this command tells how FETCH_SIZE / WRITE_SIZE read for known traffic."""
@@ -85,6 +163,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="tiny workload for a quick check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--converge", action="store_true",
+                    help="also measure the time to the converged MAP cost (plain init)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -125,6 +205,7 @@ def main():
     value = s["updates"] / el
     vox_s = s["voxel_ms"] / 1e3
     achieved = s["alg_bytes"] / vox_s / 1e9 if vox_s > 0 else 0.0
+    traffic = None if args.small else measured_traffic()
     line = {
         "metric": METRIC,
         "value": value,
@@ -151,9 +232,14 @@ def main():
                          "updates_per_s": s["updates"] / vox_s if vox_s > 0 else None,
                          "entries": s["entries"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": s["alg_bytes"] / max(s["voxel_launches"], 1),
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None},
         "cost_after": cost,
     }
+    if args.converge:
+        log("time to converged ...")
+        line["time_to_converged"] = time_to_converged(scan)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)

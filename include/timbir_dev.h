@@ -40,11 +40,13 @@ typedef struct {
   int32_t write_files;            /* write the .bin outputs (default 1) */
   uint32_t order_seed;            /* seed of the per-cell voxel order */
   int32_t verbose;                /* [INFO] lines on the debug stream */
-  int32_t kernel;                 /* voxel-update kernel: 0 = LDS-resident
-                                     multi-wavefront kernel where the cell fits
-                                     (else the next), 1 = one-voxel kernel
-                                     only, 2 = LDS-resident one-wavefront
-                                     kernel where it fits */
+  int32_t kernel;                 /* voxel-update kernel: 0 = the fastest
+                                     that applies (LDS-resident multi-, then
+                                     one-wavefront, then one-voxel), 1 = the
+                                     one-voxel kernel only, 2 = the
+                                     one-wavefront LDS kernel where it fits,
+                                     3 = the multi-wavefront LDS kernel where
+                                     it fits */
 } tb_options;
 
 void tb_get_options(tb_options *opt);

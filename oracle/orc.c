@@ -1164,18 +1164,26 @@ int32_t orc_icd_iteration(orc_problem *p, int32_t iter, double *ratio) {
 /* ======================================================================= */
 /* bilinear x2 along x then y of one plane (upsample_object_bilinear_2D,
  * XT_ICD_update.c:362-396); tmp sized Ny x 2Nx */
-static void up2d_plane(const double *in, int Ny, int Nx, double *tmp, double *out) {
+/* mid: how the x pass is stored.  The 2D upsample keeps it in a buffer of
+ * the array type (Real_arr_t, XT_ICD_update.c:362-375): MID_ARR for the
+ * object, MID_FLOAT for the float magnitude map; the 3D upsample keeps it in
+ * double (Real_t buffers, XT_ICD_update.c:398-480): MID_DOUBLE. */
+enum { MID_DOUBLE = 0, MID_ARR = 1, MID_FLOAT = 2 };
+static double mid_round(double v, int mid) {
+  return mid == MID_ARR ? (double)(ORC_ARR_T)v : mid == MID_FLOAT ? (double)(float)v : v;
+}
+static void up2d_plane(const double *in, int Ny, int Nx, double *tmp, double *out, int mid) {
   int W = 2 * Nx;
   for (int j = 0; j < Ny; j++) {
     const double *r = in + (size_t)j * Nx;
     double *t = tmp + (size_t)j * W;
-    t[0] = r[0];
-    t[1] = (3.0 * r[0] + r[1]) / 4.0;
-    t[W - 1] = r[Nx - 1];
-    t[W - 2] = (r[Nx - 2] + 3.0 * r[Nx - 1]) / 4.0;
+    t[0] = mid_round(r[0], mid);
+    t[1] = mid_round((3.0 * r[0] + r[1]) / 4.0, mid);
+    t[W - 1] = mid_round(r[Nx - 1], mid);
+    t[W - 2] = mid_round((r[Nx - 2] + 3.0 * r[Nx - 1]) / 4.0, mid);
     for (int k = 1; k < Nx - 1; k++) {
-      t[2 * k] = (r[k - 1] + 3.0 * r[k]) / 4.0;
-      t[2 * k + 1] = (3.0 * r[k] + r[k + 1]) / 4.0;
+      t[2 * k] = mid_round((r[k - 1] + 3.0 * r[k]) / 4.0, mid);
+      t[2 * k + 1] = mid_round((3.0 * r[k] + r[k + 1]) / 4.0, mid);
     }
   }
   for (int k = 0; k < W; k++) {
@@ -1219,7 +1227,7 @@ void orc_init_from_coarse(orc_problem *p, const void *prev_v, int32_t prev_nz,
   for (int f = 0; f < p->N_time; f++) {
     for (int s = 0; s < prev_nz; s++) {
       for (size_t q = 0; q < P1; q++) in[q] = prev[((size_t)f * prev_nz + s) * P1 + q];
-      up2d_plane(in, Ny, Nx, tmp, up + (size_t)s * P2);
+      up2d_plane(in, Ny, Nx, tmp, up + (size_t)s * P2, p->initICD == 3 ? MID_DOUBLE : MID_ARR);
     }
     if (p->initICD == 3) {
       up_z(up, prev_nz, P2, upz);
@@ -1239,7 +1247,8 @@ void orc_init_from_coarse(orc_problem *p, const void *prev_v, int32_t prev_nz,
       for (int f = 0; f < p->N_time; f++) {
         for (int s = 0; s < prev_blocks; s++) {
           for (size_t q = 0; q < P1; q++) in[q] = prev_mag[((size_t)f * prev_blocks + s) * P1 + q];
-          up2d_plane(in, Ny, Nx, tmp, mup + (size_t)s * P2);
+          up2d_plane(in, Ny, Nx, tmp, mup + (size_t)s * P2,
+                     prev_blocks == p->nblocks ? MID_FLOAT : MID_DOUBLE);
         }
         float *dst = p->mag + (size_t)f * p->nblocks * P2;
         if (prev_blocks == p->nblocks) {

@@ -34,11 +34,11 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1, 2], ids=["multiwave", "onevoxel", "onewave"])
+@pytest.fixture(params=[0, 1, 2, 3], ids=["auto", "onevoxel", "onewave", "multiwave"])
 def kernel(request):
-    """Voxel-update kernel: 0 = LDS-resident multi-wavefront kernel where the
-    cell fits, 1 = the one-voxel-at-a-time kernel everywhere, 2 = the
-    LDS-resident one-wavefront kernel."""
+    """Voxel-update kernel: 0 = the fastest that applies, 1 = the one-voxel
+    kernel everywhere, 2 = the one-wavefront LDS kernel, 3 = the
+    multi-wavefront LDS kernel."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -49,7 +49,7 @@ def check_kernel(g, kernel):
     """The requested kernel is the one that ran."""
     st = g.stats()
     assert st["voxel_launches"] > 0
-    if kernel in (0, 2):
+    if kernel in (0, 2, 3):
         assert st["lds_launches"] == st["voxel_launches"], st
     else:
         assert st["lds_launches"] == 0, st
@@ -74,6 +74,28 @@ def test_stage_setup_matches_oracle(mxy, mz):
     # system model and the transposed, variance-scaled weights: bit for bit
     np.testing.assert_array_equal(g.detector_response(), o.hr())
     np.testing.assert_array_equal(g.weights(), o.wgt())
+
+
+@pytest.mark.parametrize("prev_mz", [1, 2])
+def test_init_from_coarse_matches_oracle(prev_mz):
+    """initObject's multires handoff (XT_ICD_update.c:541-651): bilinear x2 in
+    x-y (initICD = 2) or x-y-z (initICD = 3) of the previous stage's object
+    and magnitude map, bit for bit."""
+    log(f"init from coarse prev_mz={prev_mz}")
+    scan = scan_small(N=32, nz=4, n_views=40)
+    o0 = pyorc.Problem(scan, mult_xy=2, mult_z=prev_mz, stage_idx=0,
+                       opt=pyorc.options(schedule=pyorc.SCHED_COLOUR))
+    x0 = random_object(o0.obj_shape, o0.mask(), seed=9)
+    m0 = np.random.default_rng(4).random(o0.obj_shape).astype(np.float32)
+    o0.obj()[...] = x0
+    o0.mag()[...] = m0
+    o1 = pyorc.Problem(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=prev_mz,
+                       opt=pyorc.options(schedule=pyorc.SCHED_COLOUR))
+    g1 = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=prev_mz)
+    o1.init_from_coarse(o0)
+    g1.init_from_coarse(x0, o0.info["N_z"], m0)
+    np.testing.assert_array_equal(g1.object(), o1.obj())
+    np.testing.assert_array_equal(g1.magmap(), o1.mag())
 
 
 def test_forward_projection_bitwise():

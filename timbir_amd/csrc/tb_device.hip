@@ -780,16 +780,16 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
   }
   HIPCHK(hipEventRecord(evp[0], st->stream));
   // kernel 0 (auto): the multi-wavefront LDS kernel, else the one-wavefront
-  // LDS kernel, else the one-voxel kernel; kernel 1: one-voxel only; kernel
-  // 2: one-wavefront LDS kernel where it applies.  The LDS kernels need the
-  // cell's (e, w) block in LDS and a view's column within 3 bins x 1 row
-  // (finest stage) or 5 bins x 2 rows.
+  // one, else the one-voxel kernel; 1: one-voxel only; 2: one-wavefront
+  // where it applies; 3: multi-wavefront where it applies.  The LDS kernels
+  // need the cell's (e, w) block in LDS and a view's column within 3 bins x 1
+  // row (finest stage) or 5 bins x 2 rows.
   int nb = 0;
   if (2 * st->mult_xy + 1 <= 3 && st->mult_z == 1) nb = 3;
   else if (2 * st->mult_xy + 1 <= 5 && st->mult_z == 2) nb = 5;
   if (st->N_r < nb) nb = 0;
   bool launched = false;
-  if (nb && g_opt.kernel == 0) {
+  if (nb && (g_opt.kernel == 0 || g_opt.kernel == 3)) {
     using mw_fn = void (*)(KP, const int2 *, int, int, int, int, int);
     const int epv = nb * st->mult_z;
     const int nth = (st->max_nv + 64 / epv - 1) / (64 / epv);
@@ -809,7 +809,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       launched = true;
     }
   }
-  if (!launched && nb && (g_opt.kernel == 0 || g_opt.kernel == 2)) {
+  if (!launched && nb && g_opt.kernel != 1) {
     using lds_fn = void (*)(KP, const int2 *, int, int, int, int);
     lds_fn fast = nullptr;
     const int vpl = (st->max_nv + 63) / 64;
