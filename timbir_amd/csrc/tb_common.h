@@ -58,6 +58,11 @@ struct KP {
   float *dummy;                // per-lane scratch word for masked-off stores
   const int *single;           // one-pixel sequence (single voxel update)
   int nobj;                    // object words; obj and mag hold 64 more
+  // A-matrix column table of the LDS kernels: per (frame, pixel, view slot)
+  // the values at bins w0 .. w0 + nb - 1 and w0 (as int bits, -1 outside
+  // the detector), colt_rec floats per record, colt_nv view slots per frame
+  const float *colt;
+  int colt_rec, colt_nv;
   double *cell_voxmag;
   unsigned long long *cell_upd, *cell_zero, *cell_ent;
   uint32_t order_seed;

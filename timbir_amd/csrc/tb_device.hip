@@ -24,6 +24,8 @@ template <int VPL, int NB, int MZ, int G>
 __global__ void k_icd_lds(KP, const int2 *, int, int, int, int);
 template <int NB, int MZ, int MAXTH>
 __global__ void k_icd_mw(KP, const int2 *, int, int, int, int, int);
+template <int NB>
+__global__ void k_build_columns(KP);
 __global__ void k_offset_partial(KP, int, int, double *, double *);
 __global__ void k_offset_finalize(KP, int, const double *, const double *, double *, double *);
 __global__ void k_offset_gamma(KP, const double *, const double *, double *);
@@ -98,6 +100,8 @@ struct tb_stage {
   int ord_cap = 0, ord_iter = -1;
   float *dummy = nullptr;
   int *single = nullptr;
+  float *colt = nullptr;  // column table of the multi-wavefront kernel (built on first use)
+  int colt_nb = 0;
   size_t lds_max = 65536;
   int2 *cells[2] = {nullptr, nullptr};
   int ncells[2] = {0, 0};
@@ -131,6 +135,7 @@ struct tb_stage {
     P.mask = mask; P.nh_list = nh_list; P.nh_count = nh_count; P.nh_cap = nh_cap;
     P.nh_ord = nh_ord; P.nh_ord_count = nh_ord_count; P.ord = ord; P.ord_off = ord_off;
     P.ord_cap = ord_cap; P.dummy = dummy; P.single = single; P.nobj = (int)nobj();
+    P.colt = colt; P.colt_nv = max_nv; P.colt_rec = colt_nb == 5 ? 8 : 4;
     P.u_inv = 1.0 / delta_r; P.u_r0 = R0 / delta_r; P.u_dxy = delta_xy / delta_r;
     P.u_max = (RMax - R0) / delta_r; P.u_kk = delta_r / OffsetR; P.mxy = mult_xy;
     {
@@ -205,7 +210,7 @@ extern "C" void tb_stage_destroy(tb_stage *st) {
   void *bufs[] = {st->e, st->w, st->y, st->d, st->obj, st->mag, st->hr, st->filt, st->cosv,
                   st->sinv, st->sfilt_d, st->view_off_d, st->view_idx_d, st->view_frame_d,
                   st->mask, st->nh_list, st->nh_count, st->nh_ord, st->nh_ord_count, st->ord,
-                  st->ord_off, st->dummy, st->single, st->cells[0], st->cells[1],
+                  st->ord_off, st->dummy, st->single, st->colt, st->cells[0], st->cells[1],
                   st->cell_voxmag, st->cell_upd, st->cell_zero, st->cell_ent, st->off_pnum,
                   st->off_pden, st->off_num, st->off_den, st->off_gamma, st->red};
   for (void *b : bufs)
@@ -769,6 +774,37 @@ static int ensure_order(tb_stage *st, int iter) {
   return 0;
 }
 
+// The A-matrix column table of the multi-wavefront kernel: it depends on the
+// stage's geometry only, so it is built once, by a kernel over every
+// (frame, pixel, view slot).  Returns 0 when the table is ready, 1 when it
+// would not fit (the caller then uses another kernel), -1 on a HIP error.
+static int ensure_columns(tb_stage *st, int nb) {
+  if (st->colt && st->colt_nb == nb) return 0;
+  if (st->colt) {
+    HIPCHK(hipStreamSynchronize(st->stream));
+    HIPCHK(hipFree(st->colt));
+    st->colt = nullptr;
+  }
+  const int rec = nb == 5 ? 8 : 4;
+  const size_t n = (size_t)st->N_time * st->nxy() * st->max_nv * rec;
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  if (n * sizeof(float) > free_b / 4) return 1;
+  if (dalloc(&st->colt, n) != hipSuccess) {
+    (void)hipGetLastError();
+    st->colt = nullptr;
+    return 1;
+  }
+  st->colt_nb = nb;
+  KP P = st->kp();
+  const size_t recs = n / rec;
+  const int grid = (int)std::min<size_t>((recs + 255) / 256, 65536);
+  if (nb == 5) hipLaunchKernelGGL(k_build_columns<5>, dim3(grid), dim3(256), 0, st->stream, P);
+  else hipLaunchKernelGGL(k_build_columns<3>, dim3(grid), dim3(256), 0, st->stream, P);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk,
                          int seq, int pix, int count_mag) {
   if (ncells == 0) return 0;
@@ -799,7 +835,14 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
                     : nth <= 15 ? k_icd_mw<3, 1, 15> : nullptr;
     else fn = nth <= 8 ? k_icd_mw<5, 2, 8> : nth <= 15 ? k_icd_mw<5, 2, 15> : nullptr;
     const size_t lds = icd_mw_lds_bytes(st->max_nv, st->N_r, st->mult_z, nth);
-    if (fn && lds + 1024 <= st->lds_max) {
+    int ec = 1;
+    if (fn && lds + 1024 <= st->lds_max) ec = ensure_columns(st, nb);
+    if (ec < 0) return -1;
+    if (ec == 0) {
+      // the table may have been allocated just now: take the parameters again
+      P = st->kp();
+      if (!P.colt || P.colt_rec != (nb == 5 ? 8 : 4) || P.colt_nv != st->max_nv)
+        return fail("launch_update: column table missing");
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

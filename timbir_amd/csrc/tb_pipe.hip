@@ -396,15 +396,25 @@ template <int MAXTH>
 __device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int par, int nth,
                                             float *V_out, bool *skip_out) {
 #pragma clang fp contract(off)
-  // every slot is read at once (one LDS latency); slots of absent waves
-  // hold zero
-  double T1 = 0.0;
-  float T2 = 0.0f;
+  // every slot is read at once (one LDS latency) and added as a fixed
+  // pairwise tree; slots of absent waves hold zero
+  constexpr int N2 = MAXTH <= 4 ? 4 : MAXTH <= 8 ? 8 : 16;
+  double a1[N2];
+  float a2[N2];
 #pragma unroll
-  for (int w = 0; w < MAXTH; w++) {
-    T1 += S.t1[par][w];
-    T2 += S.t2[par][w];
+  for (int w = 0; w < N2; w++) {
+    a1[w] = w < MAXTH ? S.t1[par][w] : 0.0;
+    a2[w] = w < MAXTH ? S.t2[par][w] : 0.0f;
   }
+#pragma unroll
+  for (int h = N2 / 2; h >= 1; h /= 2)
+#pragma unroll
+    for (int w = 0; w < h; w++) {
+      a1[w] = a1[w] + a1[w + h];
+      a2[w] = a2[w] + a2[w + h];
+    }
+  const double T1 = a1[0];
+  const float T2 = a2[0];
   (void)nth;
   const float pb = S.pr[par][0], pbx = S.pr[par][1], V = S.pr[par][2];
   const bool skip = S.pr[par][3] != 0.0f;
@@ -415,6 +425,66 @@ __device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int pa
   *skip_out = skip;
   return skip ? V : V + step;
 }
+
+// Column table of the multi-wavefront kernel: for every (frame, pixel, view
+// slot) the A-matrix values at bins w0 .. w0 + NB - 1 of that view
+// (calcAMatrixColumnforAngle, XT_AMatrix.c:50-110, in the bin-unit form
+// above) and w0, or w0 = -1 when the voxel misses the detector in that view
+// or the slot holds no view.  Records of 4 floats (NB = 3) or 8 (NB = 5).
+template <int NB>
+__global__ __launch_bounds__(256) void k_build_columns(KP P) {
+  constexpr int REC = NB == 5 ? 8 : 4;
+  const int Nxy = P.N_x * P.N_y;
+  const size_t nrec = (size_t)P.N_time * Nxy * P.colt_nv;
+  float *out = const_cast<float *>(P.colt);
+  const int m = P.mxy;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < nrec;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int q = (int)(idx % P.colt_nv);
+    const size_t fp = idx / P.colt_nv;
+    const int lin = (int)(fp % Nxy), f = (int)(fp / Nxy);
+    const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+    float a[NB];
+    int w0 = -1;
+#pragma unroll
+    for (int ub = 0; ub < NB; ub++) a[ub] = 0.0f;
+    if (q < nv) {
+      const int v = P.view_idx[vb + q];
+      const double cs = P.cosv[v], sn = P.sinv[v];
+      const double Au = P.delta_xy * cs * P.u_inv, Bu = P.delta_xy * sn * P.u_inv;
+      const double Cu =
+          ((P.x0 + 0.5 * P.delta_xy) * cs - (P.y0 + 0.5 * P.delta_xy) * sn - P.R0) * P.u_inv;
+      const int yy = lin / P.N_x, xx = lin - yy * P.N_x;
+      const double u = fma((double)xx, Au, fma(-(double)yy, Bu, Cu));
+      if (u + m >= 0.0 && u - m <= P.u_max) {
+        const int lo = (int)floor(u) - m;
+        const int loc = max(lo, 0), hic = min(lo + 2 * m, P.N_r - 1);
+        w0 = min(loc, P.N_r - NB);
+        const float *H = P.hr + (size_t)v * kHrStride;
+#pragma unroll
+        for (int ub = 0; ub < NB; ub++) {
+          const int j = w0 + ub;
+          const double t = fabs(u - ((double)j + 0.5)) * P.u_kk;
+          const int bin = (int)t;
+          const bool ok = (j >= loc) & (j <= hic) & (bin < kRespBins);
+          const int bb = ok ? bin : 0;
+          const float fr = (float)(t - (double)bin);
+          const float h0 = H[bb], h1 = H[bb + 1];
+          a[ub] = fmaf(fr, h1 - h0, h0) * (ok ? 1.0f : 0.0f);
+        }
+      }
+    }
+    float *r = out + idx * REC;
+#pragma unroll
+    for (int ub = 0; ub < NB; ub++) r[ub] = a[ub];
+    r[NB] = __int_as_float(w0);
+#pragma unroll
+    for (int k = NB + 1; k < REC; k++) r[k] = 0.0f;
+  }
+}
+
+template __global__ void k_build_columns<3>(KP);
+template __global__ void k_build_columns<5>(KP);
 
 template <int NB, int MZ, int MAXTH>
 __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ cells, int iter,
@@ -470,83 +540,142 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
 
   if (wave < nth) {
     // ---------------- theta wave ----------------
+    // The lane's entry is (view q, bin w0 + ub, row mm); its A value and w0
+    // come from the column table, prefetched one group of G voxels ahead.
+    // The next voxel's (e, w) is read before this voxel's e is written and
+    // corrected afterwards by a(this voxel, that bin) * dx: the same fma the
+    // lane owning that entry stores, so no bits change.
+    constexpr int REC = NB == 5 ? 8 : 4;
     const int lq = lane / EPV, ent = lane - lq * EPV;
     const int ub = ent / MZ, mm = ent - ub * MZ;
     const int q = wave * vpw + lq;
     const bool active = lq < vpw && q < nv;
     const int qv = active ? q : 0;
-    const int v = P.view_idx[vb + qv];
-    const double c = P.cosv[v], sn = P.sinv[v];
-    const double Au = P.delta_xy * c * P.u_inv, Bu = P.delta_xy * sn * P.u_inv;
-    const double Cu =
-        ((P.x0 + 0.5 * P.delta_xy) * c - (P.y0 + 0.5 * P.delta_xy) * sn - P.R0) * P.u_inv;
-    const int m = P.mxy;
     const int dummy = rows + wave * 64 + lane;
-    const float *H = s_h + qv * kHrStride;
-
-    // the entry of voxel `pix` this lane owns: value and LDS index
-    auto column = [&](int pix, float &a, int &at) {
+    const float4 *colt = reinterpret_cast<const float4 *>(P.colt);
+    const size_t fbase = (size_t)f * Nxy;
+    struct Rec {
+      float a[NB];
+      int w0;
+    };
+    auto load = [&](int pix) -> Rec {
       const bool valid = pix != kNoPix;
-      const int yy = valid ? (pix >> 16) : 0, xx = valid ? (pix & 0xffff) : 0;
-      const double u = fma((double)xx, Au, fma(-(double)yy, Bu, Cu));
-      const bool inr = valid & active & (u + m >= 0.0) & (u - m <= P.u_max);
-      const int lo = (int)floor(u) - m;
-      const int loc = max(lo, 0), hic = min(lo + 2 * m, P.N_r - 1);
-      const int w0 = min(loc, P.N_r - NB);
-      const int j = w0 + ub;
-      const double t = fabs(u - ((double)j + 0.5)) * P.u_kk;
-      const int bin = (int)t;
-      const bool ok = inr & (j >= loc) & (j <= hic) & (bin < kRespBins);
-      const int bb = ok ? bin : 0;
-      const float fr = (float)(t - (double)bin);
-      const float h0 = H[bb], h1 = H[bb + 1];
-      a = fmaf(fr, h1 - h0, h0) * (ok ? 1.0f : 0.0f);
-      at = inr ? qv * RS + j * MZ + mm : dummy;
+      const int lin = valid ? (pix >> 16) * P.N_x + (pix & 0xffff) : 0;
+      const size_t r = ((fbase + lin) * P.colt_nv + qv) * (REC / 4);
+      Rec o;
+      const float4 r0 = colt[r];
+      if (REC == 4) {
+        o.a[0] = r0.x;
+        o.a[1] = r0.y;
+        o.a[2] = r0.z;
+        o.w0 = __float_as_int(r0.w);
+      } else {
+        const float4 r1 = colt[r + 1];
+        const float v[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+#pragma unroll
+        for (int k = 0; k < NB; k++) o.a[k] = v[k];
+        o.w0 = __float_as_int(v[NB]);
+      }
+      // no arithmetic on the loaded registers here: any would make the
+      // compiler wait for the load at once; validity is applied at use
+      return o;
     };
-
-    // list window: entries [wb, wb + 64) in w0r; the next 64 are in flight
-    // in w1r (loaded 64 voxels before they are needed)
-    int wb = 0;
-    int w0r = list[n0 + min(lane, cnt - 1)];
-    int w1r = list[n0 + min(64 + lane, cnt - 1)];
-    auto pix_at = [&](int n) -> int {
-      return n < cnt ? __builtin_amdgcn_readlane(w0r, n - wb) : kNoPix;
+    auto pick = [&](const Rec &r, int k) -> float {
+      float v = 0.0f;
+#pragma unroll
+      for (int t = 0; t < NB; t++) v = k == t ? r.a[t] : v;
+      return v;
     };
+    // w0 of a record as this lane uses it: -1 for an empty slot or a lane
+    // without a view
+    auto w0_of = [&](const Rec &r, int pix) -> int {
+      return (active & (pix != kNoPix)) ? r.w0 : -1;
+    };
+    auto entry_at = [&](int w0) -> int { return w0 >= 0 ? qv * RS + (w0 + ub) * MZ + mm : dummy; };
 
-    float a, an;
-    int at, atn;
-    column(pix_at(0), a, at);
+    int apix[G], bpix[G];
+    Rec arec[G], brec[G];
+    const int ngroups = (cnt + G - 1) / G;
+    int raw = list[n0 + min(lane, cnt - 1)];
+    int pnext[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) pnext[k] = k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      apix[k] = kNoPix;
+      arec[k].w0 = -1;
+#pragma unroll
+      for (int t = 0; t < NB; t++) arec[k].a[t] = 0.0f;
+    }
+    // current voxel's entry, its (e, w) read ahead, and the correction
+    float ca = 0.0f, cprev = 0.0f, dxprev = 0.0f;
+    int cat = dummy, cw0 = -1;
+    float2 ews = make_float2(0.0f, 0.0f);
+    Rec crec = arec[0];
     unsigned long long nent = 0;
-    for (int n = 0; n < cnt; n++) {
-      const int par = n & 1;
-      const float2 ew = s_ew[at];
-      const float e = ew.x, w = ew.y;
-      const float cw_out = P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
-      const float cw = huber_sel(P, e, w) ? w : cw_out;
-      const float ac = a * cw;
-      double t1 = (double)(-e * ac);
-      float t2 = a * ac;
-      wave_total_df(t1, t2);
-      if (lane == 0) {
-        S.t1[par][wave] = t1;
-        S.t2[par][wave] = t2;
+    for (int g = 0; g <= ngroups; g++) {
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        bpix[k] = pnext[k];
+        brec[k] = load(pnext[k]);
       }
-      nent += a > 0.0f ? 1 : 0;
-      // next voxel's entry while the other waves finish
-      if (n + 1 - wb >= 64) {
-        wb += 64;
-        w0r = w1r;
-        w1r = list[n0 + min(wb + 64 + lane, cnt - 1)];
+      raw = list[n0 + min((g + 1) * G + lane, cnt - 1)];
+#pragma unroll
+      for (int s = 0; s < G; s++) {
+        const int pix = apix[s];
+        if (pix == kNoPix) continue;  // only past the end of the sequence
+        const int n = (g - 1) * G + s;
+        const int par = n & 1;
+        const float e = fmaf(-cprev, dxprev, ews.x), w = ews.y;
+        const float cw_out =
+            P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
+        const float cw = huber_sel(P, e, w) ? w : cw_out;
+        const float ac = ca * cw;
+        double t1 = (double)(-e * ac);
+        float t2 = ca * ac;
+        wave_total_df(t1, t2);
+        if (lane == 0) {
+          S.t1[par][wave] = t1;
+          S.t2[par][wave] = t2;
+        }
+        nent += ca > 0.0f ? 1 : 0;
+        // next voxel's entry and its (e, w) while the other waves finish
+        const Rec &nr = s + 1 < G ? arec[s + 1] : brec[0];
+        const int nw0 = w0_of(nr, s + 1 < G ? apix[s + 1] : bpix[0]);
+        const int nat = entry_at(nw0);
+        const float2 ewn = s_ew[nat];
+        const int d = nw0 + ub - cw0;  // the next voxel's bin in this voxel's window
+        const float cn_prev = (nw0 >= 0 && cw0 >= 0) ? pick(crec, d) : 0.0f;
+        __syncthreads();
+        float V;
+        bool skip;
+        const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
+        const float dx = xn - V;
+        s_ew[cat].x = fmaf(-ca, dx, e);
+        crec = nr;
+        cw0 = nw0;
+        ca = nw0 >= 0 ? pick(nr, ub) : 0.0f;
+        cat = nat;
+        ews = ewn;
+        cprev = cn_prev;
+        dxprev = dx;
       }
-      column(pix_at(n + 1), an, atn);
-      __syncthreads();
-      float V;
-      bool skip;
-      const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
-      const float dx = xn - V;
-      s_ew[at].x = fmaf(-a, dx, e);
-      a = an;
-      at = atn;
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        apix[k] = bpix[k];
+        arec[k] = brec[k];
+      }
+#pragma unroll
+      for (int k = 0; k < G; k++)
+        pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+      if (g == 0) {
+        // the first voxel's entry
+        crec = arec[0];
+        cw0 = w0_of(crec, apix[0]);
+        ca = cw0 >= 0 ? pick(crec, ub) : 0.0f;
+        cat = entry_at(cw0);
+        ews = s_ew[cat];
+      }
     }
     const double ne = wave_sum_d((double)nent);
     if (lane == 0) S.nent[wave] = (unsigned long long)ne;
