@@ -134,7 +134,7 @@ def time_to_converged(scan, finest_cap: int = 400, tol: float = 1e-4):
         if not finest:
             st.synchronize()
             prev = dict(obj=st.object(), nz=st.info["N_z"], mag=st.magmap(), off=st.offset())
-            var_est = st.var_est()
+            var_est = st.var_est
         st.close()
     cstar = trace[-1][1]
     hit = next((t for t, c in trace if abs(c - cstar) <= tol * abs(cstar)), None)
