@@ -331,3 +331,31 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
         assert rmse < 1e-4
         assert abs(cg - co) <= 1e-5 * abs(co)
     check_kernel(g, kernel)
+
+
+def test_fullsize_stage_matches_oracle():
+    """The full 4D interlaced bench stage (256x256x8 slices x 16 frames, 1024
+    views, 256 bins, zingers and rings, stiff prior): MAP cost after ICD
+    iterations 1..3 within 1e-5 of the CPU oracle's (cached in
+    tests/golden/fullsize_interlaced4d.json by make_fullsize.py), object sum
+    and L2 norm within 1e-4."""
+    import json
+    import os
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "fullsize_interlaced4d.json")))
+    log("full-size stage vs cached oracle")
+    scan = datagen.interlaced4d()
+    assert datagen.sha256(scan["projections"]) == ref["scan_sha256"], "scan generator changed"
+    g = tb.Stage(scan, **ref["stage"])
+    g.init_error_sino()
+    for it in range(1, ref["iterations"] + 1):
+        g.icd_iteration(it)
+        c = g.cost()[0]
+        co = ref["costs"][it - 1]
+        log(f"  it {it}: cost {c:.10e} vs {co:.10e} ({(c - co) / co:.2e})")
+        assert abs(c - co) <= 1e-5 * abs(co)
+    x = g.object().astype(np.float64)
+    assert abs(x.sum() - ref["obj_sum"]) <= 1e-4 * abs(ref["obj_sum"])
+    assert abs(np.sqrt((x * x).sum()) - ref["obj_l2"]) <= 1e-4 * ref["obj_l2"]
+    g.close()
+

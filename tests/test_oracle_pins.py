@@ -327,3 +327,18 @@ def test_multires_upsampling_is_bilinear():
     expect = 0.3 * (fx / 2 - 0.25) + 0.2 * (fy / 2 - 0.25) + 0.1 * (fz / 2 - 0.25)
     inner = (slice(1, -1), slice(1, -1), slice(1, -1))
     np.testing.assert_allclose(out[inner], expect[inner], atol=1e-5)
+
+
+def test_fullsize_fixture_matches_generator():
+    """The cached full-size oracle results belong to the scan the generator
+    makes today (seeded, so the hash is fixed)."""
+    import json
+    import os
+    from timbir_amd import datagen
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "fullsize_interlaced4d.json")))
+    scan = datagen.interlaced4d()
+    assert datagen.sha256(scan["projections"]) == ref["scan_sha256"]
+    assert len(ref["costs"]) == ref["iterations"] == 3
+    assert ref["costs"][0] > ref["costs"][1] > ref["costs"][2]
+
