@@ -41,12 +41,16 @@ typedef struct {
   uint32_t order_seed;            /* seed of the per-cell voxel order */
   int32_t verbose;                /* [INFO] lines on the debug stream */
   int32_t kernel;                 /* voxel-update kernel: 0 = the fastest
-                                     that applies (LDS-resident multi-, then
-                                     one-wavefront, then one-voxel), 1 = the
-                                     one-voxel kernel only, 2 = the
-                                     one-wavefront LDS kernel where it fits,
-                                     3 = the multi-wavefront LDS kernel where
-                                     it fits */
+                                     that applies (LDS-resident multi-
+                                     wavefront, then multi-wavefront with e
+                                     in global memory, then one-wavefront,
+                                     then one-voxel), 1 = the one-voxel
+                                     kernel only, 2 = the one-wavefront LDS
+                                     kernel where it fits, 3 = the
+                                     multi-wavefront LDS kernel where it
+                                     fits, 4 = the multi-wavefront kernel
+                                     with e in global memory where it
+                                     applies */
 } tb_options;
 
 void tb_get_options(tb_options *opt);
@@ -139,8 +143,9 @@ int tb_stage_run(tb_stage *st, int32_t max_iter, double *costs,
  * voxel updates performed (zero-skipped voxels excluded), and the
  * algorithmic bytes they moved (12 B per touched A-matrix entry: e read,
  * w read, e write).  Other kernels' time in other_ms.  lds_launches counts
- * the voxel-update launches that ran the LDS-resident kernel (the others ran
- * the one-voxel-at-a-time kernel). */
+ * the voxel-update launches that ran an LDS-resident kernel, global_launches
+ * those that ran the multi-wavefront kernel with e in global memory (the
+ * others ran the one-voxel-at-a-time kernel). */
 typedef struct {
   double voxel_ms;
   double other_ms;
@@ -150,6 +155,7 @@ typedef struct {
   int64_t entries;
   double alg_bytes;
   int64_t lds_launches;
+  int64_t global_launches;
 } tb_stats;
 int tb_get_stats(tb_stage *st, tb_stats *out);
 int tb_reset_stats(tb_stage *st);

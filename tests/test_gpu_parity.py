@@ -34,11 +34,13 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1, 2, 3], ids=["auto", "onevoxel", "onewave", "multiwave"])
+@pytest.fixture(params=[0, 1, 2, 3, 4],
+                ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal"])
 def kernel(request):
     """Voxel-update kernel: 0 = the fastest that applies, 1 = the one-voxel
     kernel everywhere, 2 = the one-wavefront LDS kernel, 3 = the
-    multi-wavefront LDS kernel."""
+    multi-wavefront LDS kernel, 4 = the multi-wavefront kernel with e in
+    global memory."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -49,10 +51,14 @@ def check_kernel(g, kernel):
     """The requested kernel is the one that ran."""
     st = g.stats()
     assert st["voxel_launches"] > 0
-    if kernel in (0, 2, 3):
+    if kernel == 0:
+        assert st["lds_launches"] + st["global_launches"] == st["voxel_launches"], st
+    elif kernel in (2, 3):
         assert st["lds_launches"] == st["voxel_launches"], st
+    elif kernel == 4:
+        assert st["global_launches"] == st["voxel_launches"], st
     else:
-        assert st["lds_launches"] == 0, st
+        assert st["lds_launches"] == 0 and st["global_launches"] == 0, st
 
 
 def random_object(shape, mask, seed=1, scale=0.02):
@@ -308,11 +314,17 @@ def test_phases_track_oracle(kernel, c=1.0):
     (1, 1, 200, 2),   # 100 views per frame: two per lane
     (2, 2, 40, 2),    # second-finest stage: 5 bins x 2 rows per view
     (2, 2, 200, 2),
+    (4, 4, 40, 2),    # 9 bins x 4 rows per view (e in global memory only)
+    (4, 4, 200, 2),   # 100 views per frame: 15 theta waves
+    (4, 4, 300, 2),   # 150 views per frame: two views per lane
 ])
 def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
     """Every instantiation of the voxel-update kernels follows the oracle."""
+    if mxy == 4 and kernel in (2, 3):
+        pytest.skip("the LDS kernels take columns of at most 5 bins x 2 rows")
     log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
-    scan = scan_small(N=32, nz=4, n_views=n_views, n_frames=n_frames, c_s=1.0, c_t=1.0)
+    n, nz = (64, 8) if mxy == 4 else (32, 4)
+    scan = scan_small(N=n, nz=nz, n_views=n_views, n_frames=n_frames, c_s=1.0, c_t=1.0)
     stage = 1 if mxy > 1 else 0
     o, g = pair(scan, stage_idx=stage, mult_xy=mxy, mult_z=mz,
                 prev_mult_z=2 * mz if stage else 0)

@@ -39,7 +39,8 @@ class TbStats(C.Structure):
     _fields_ = [("voxel_ms", C.c_double), ("other_ms", C.c_double),
                 ("voxel_launches", C.c_int64), ("updates", C.c_int64),
                 ("zero_skips", C.c_int64), ("entries", C.c_int64), ("alg_bytes", C.c_double),
-                ("lds_launches", C.c_int64)]
+                ("lds_launches", C.c_int64),
+                ("global_launches", C.c_int64)]
 
 
 class TimbirError(RuntimeError):

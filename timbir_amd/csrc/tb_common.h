@@ -128,4 +128,8 @@ TB_HD uint32_t perm(uint32_t n, uint32_t N, uint32_t key) {
   return n;
 }
 
+// Floats per column-table record for columns of nb bins: nb values and w0,
+// rounded up to whole 16-byte words.
+TB_HD constexpr int col_rec(int nb) { return (nb + 4) / 4 * 4; }
+
 }  // namespace tb

@@ -24,6 +24,8 @@ template <int VPL, int NB, int MZ, int G>
 __global__ void k_icd_lds(KP, const int2 *, int, int, int, int);
 template <int NB, int MZ, int MAXTH>
 __global__ void k_icd_mw(KP, const int2 *, int, int, int, int, int);
+template <int NB, int MZ, int VPL, int MAXTH>
+__global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
 template <int NB>
 __global__ void k_build_columns(KP);
 __global__ void k_offset_partial(KP, int, int, double *, double *);
@@ -116,7 +118,7 @@ struct tb_stage {
   std::vector<hipEvent_t> ev;
   int ev_used = 0;
   double voxel_ms = 0, other_ms = 0;
-  long long voxel_launches = 0, lds_launches = 0;
+  long long voxel_launches = 0, lds_launches = 0, global_launches = 0;
   hipEvent_t o0 = nullptr, o1 = nullptr;
 
   KP kp() const {
@@ -135,7 +137,7 @@ struct tb_stage {
     P.mask = mask; P.nh_list = nh_list; P.nh_count = nh_count; P.nh_cap = nh_cap;
     P.nh_ord = nh_ord; P.nh_ord_count = nh_ord_count; P.ord = ord; P.ord_off = ord_off;
     P.ord_cap = ord_cap; P.dummy = dummy; P.single = single; P.nobj = (int)nobj();
-    P.colt = colt; P.colt_nv = max_nv; P.colt_rec = colt_nb == 5 ? 8 : 4;
+    P.colt = colt; P.colt_nv = max_nv; P.colt_rec = colt_nb ? col_rec(colt_nb) : 0;
     P.u_inv = 1.0 / delta_r; P.u_r0 = R0 / delta_r; P.u_dxy = delta_xy / delta_r;
     P.u_max = (RMax - R0) / delta_r; P.u_kk = delta_r / OffsetR; P.mxy = mult_xy;
     {
@@ -785,7 +787,7 @@ static int ensure_columns(tb_stage *st, int nb) {
     HIPCHK(hipFree(st->colt));
     st->colt = nullptr;
   }
-  const int rec = nb == 5 ? 8 : 4;
+  const int rec = col_rec(nb);
   const size_t n = (size_t)st->N_time * st->nxy() * st->max_nv * rec;
   size_t free_b = 0, total_b = 0;
   HIPCHK(hipMemGetInfo(&free_b, &total_b));
@@ -799,7 +801,9 @@ static int ensure_columns(tb_stage *st, int nb) {
   KP P = st->kp();
   const size_t recs = n / rec;
   const int grid = (int)std::min<size_t>((recs + 255) / 256, 65536);
-  if (nb == 5) hipLaunchKernelGGL(k_build_columns<5>, dim3(grid), dim3(256), 0, st->stream, P);
+  if (nb == 9) hipLaunchKernelGGL(k_build_columns<9>, dim3(grid), dim3(256), 0, st->stream, P);
+  else if (nb == 5)
+    hipLaunchKernelGGL(k_build_columns<5>, dim3(grid), dim3(256), 0, st->stream, P);
   else hipLaunchKernelGGL(k_build_columns<3>, dim3(grid), dim3(256), 0, st->stream, P);
   HIPCHK(hipGetLastError());
   return 0;
@@ -815,18 +819,38 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     evp = ev_pair(st);
   }
   HIPCHK(hipEventRecord(evp[0], st->stream));
-  // kernel 0 (auto): the multi-wavefront LDS kernel, else the one-wavefront
-  // one, else the one-voxel kernel; 1: one-voxel only; 2: one-wavefront
-  // where it applies; 3: multi-wavefront where it applies.  The LDS kernels
-  // need the cell's (e, w) block in LDS and a view's column within 3 bins x 1
-  // row (finest stage) or 5 bins x 2 rows.
+  // kernel 0 (auto): the multi-wavefront LDS kernel, else the
+  // multi-wavefront kernel with e in global memory, else the one-wavefront
+  // LDS kernel, else the one-voxel kernel; 1: one-voxel only; 2:
+  // one-wavefront where it applies; 3: multi-wavefront LDS where it applies;
+  // 4: multi-wavefront global where it applies.  The LDS kernels need the
+  // cell's (e, w) block in LDS and a view's column within 3 bins x 1 row
+  // (finest stage) or 5 bins x 2 rows.
+  using mw_fn = void (*)(KP, const int2 *, int, int, int, int, int);
   int nb = 0;
   if (2 * st->mult_xy + 1 <= 3 && st->mult_z == 1) nb = 3;
   else if (2 * st->mult_xy + 1 <= 5 && st->mult_z == 2) nb = 5;
   if (st->N_r < nb) nb = 0;
   bool launched = false;
+  auto run_mw = [&](mw_fn fn, int colnb, int nth, int vpw, size_t lds, bool global) -> int {
+    // the column table may be allocated now: take the parameters again
+    const int ec = ensure_columns(st, colnb);
+    if (ec != 0) return ec;
+    P = st->kp();
+    if (!P.colt || P.colt_rec != col_rec(colnb) || P.colt_nv != st->max_nv)
+      return fail("launch_update: column table missing");
+    if (seq == 0 && ensure_order(st, iter)) return -1;
+    if (lds)
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
+                       chunk, seq, count_mag, vpw);
+    if (global) st->global_launches++;
+    else st->lds_launches++;
+    launched = true;
+    return 0;
+  };
   if (nb && (g_opt.kernel == 0 || g_opt.kernel == 3)) {
-    using mw_fn = void (*)(KP, const int2 *, int, int, int, int, int);
     const int epv = nb * st->mult_z;
     const int nth = (st->max_nv + 64 / epv - 1) / (64 / epv);
     const int vpw = (st->max_nv + nth - 1) / nth;
@@ -835,22 +859,31 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
                     : nth <= 15 ? k_icd_mw<3, 1, 15> : nullptr;
     else fn = nth <= 8 ? k_icd_mw<5, 2, 8> : nth <= 15 ? k_icd_mw<5, 2, 15> : nullptr;
     const size_t lds = icd_mw_lds_bytes(st->max_nv, st->N_r, st->mult_z, nth);
-    int ec = 1;
-    if (fn && lds + 1024 <= st->lds_max) ec = ensure_columns(st, nb);
-    if (ec < 0) return -1;
-    if (ec == 0) {
-      // the table may have been allocated just now: take the parameters again
-      P = st->kp();
-      if (!P.colt || P.colt_rec != (nb == 5 ? 8 : 4) || P.colt_nv != st->max_nv)
-        return fail("launch_update: column table missing");
-      if (seq == 0 && ensure_order(st, iter)) return -1;
-      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
-                         chunk, seq, count_mag, vpw);
-      st->lds_launches++;
-      launched = true;
+    if (fn && lds + 1024 <= st->lds_max && run_mw(fn, nb, nth, vpw, lds, false) < 0) return -1;
+  }
+  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 4)) {
+    // lane = (view, bin) with the slice's mult_z rows: NB bins, VW = 64 / NB
+    // views per lane layer, VPL layers
+    const int w = 2 * st->mult_xy + 1;
+    const int gnb = w <= 3 ? 3 : w <= 5 ? 5 : w <= 9 ? 9 : 0;
+    const int mz = st->mult_z;
+    int vpl = 0;
+    mw_fn fn = nullptr;
+    int nth = 0, vpw = 0;
+    for (int tryv = 1; tryv <= 2 && !fn && gnb && st->N_r >= gnb; tryv++) {
+      const int cap = (64 / gnb) * tryv;
+      nth = (st->max_nv + cap - 1) / cap;
+      vpw = (st->max_nv + nth - 1) / nth;
+      vpl = tryv;
+      if (nth > 15) continue;
+      if (gnb == 3 && mz == 1 && vpl == 1)
+        fn = nth <= 8 ? k_icd_mwg<3, 1, 1, 8> : k_icd_mwg<3, 1, 1, 15>;
+      else if (gnb == 5 && mz == 2 && vpl == 1)
+        fn = nth <= 8 ? k_icd_mwg<5, 2, 1, 8> : k_icd_mwg<5, 2, 1, 15>;
+      else if (gnb == 9 && mz == 4)
+        fn = vpl == 1 ? k_icd_mwg<9, 4, 1, 15> : k_icd_mwg<9, 4, 2, 15>;
     }
+    if (fn && run_mw(fn, gnb, nth, vpw, 0, true) < 0) return -1;
   }
   if (!launched && nb && g_opt.kernel != 1) {
     using lds_fn = void (*)(KP, const int2 *, int, int, int, int);
@@ -1035,6 +1068,7 @@ extern "C" int tb_get_stats(tb_stage *st, tb_stats *out) {
   out->other_ms = st->other_ms;
   out->voxel_launches = st->voxel_launches;
   out->lds_launches = st->lds_launches;
+  out->global_launches = st->global_launches;
   out->alg_bytes = 12.0 * (double)out->entries;
   return 0;
 }
@@ -1054,6 +1088,7 @@ extern "C" int tb_reset_stats(tb_stage *st) {
   st->other_ms = 0;
   st->voxel_launches = 0;
   st->lds_launches = 0;
+  st->global_launches = 0;
   return 0;
 }
 

@@ -430,10 +430,11 @@ __device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int pa
 // slot) the A-matrix values at bins w0 .. w0 + NB - 1 of that view
 // (calcAMatrixColumnforAngle, XT_AMatrix.c:50-110, in the bin-unit form
 // above) and w0, or w0 = -1 when the voxel misses the detector in that view
-// or the slot holds no view.  Records of 4 floats (NB = 3) or 8 (NB = 5).
+// or the slot holds no view.  Records of col_rec(NB) floats (NB + 1 rounded up
+// to a multiple of 4: 4, 8, 12 for NB = 3, 5, 9).
 template <int NB>
 __global__ __launch_bounds__(256) void k_build_columns(KP P) {
-  constexpr int REC = NB == 5 ? 8 : 4;
+  constexpr int REC = col_rec(NB);
   const int Nxy = P.N_x * P.N_y;
   const size_t nrec = (size_t)P.N_time * Nxy * P.colt_nv;
   float *out = const_cast<float *>(P.colt);
@@ -485,6 +486,154 @@ __global__ __launch_bounds__(256) void k_build_columns(KP P) {
 
 template __global__ void k_build_columns<3>(KP);
 template __global__ void k_build_columns<5>(KP);
+template __global__ void k_build_columns<9>(KP);
+
+// The prior wave of the multi-wavefront kernels: the object side of the
+// voxel sequence (neighbour prefetch one group ahead with in-flight patching,
+// the qGGMRF prior surrogate of XT_ICD_update.c:134-183, zero-skip,
+// XT_VoxUpdate.c:236-259, the object and update-map stores), meeting the
+// theta waves at one barrier per voxel.
+template <int MAXTH>
+__device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *list, int n0,
+                                           int cnt, int f, int z, int iter, int seq,
+                                           int count_mag, int nth, int lane) {
+  constexpr int G = 8;
+  const int cell = f * P.N_z + z;
+  const int Nxy = P.N_x * P.N_y;
+
+  int ndz = 0, ndy = 0, ndx = 0, ndt = 0;
+  float ng = 0.0f, nsq = 1.0f, nisqp = 1.0f, nc = 1.0f;
+  const bool nlane = lane < 29;
+  if (nlane) {
+    if (lane < 27) {
+      ndz = lane / 9 - 1;
+      ndy = (lane / 3) % 3 - 1;
+      ndx = lane % 3 - 1;
+      ng = P.sfilt[lane];
+      nsq = P.sSq, nisqp = 1.0f / P.sSqp, nc = P.c_s;
+    } else {
+      ndt = lane == 27 ? -1 : 1;
+      ng = P.tfilt;
+      nsq = P.sTq, nisqp = 1.0f / P.sTqp, nc = P.c_t;
+    }
+  }
+  const bool inplane = lane < 27 && ndz == 0 && lane != 13;
+  const int noff = ndy * 65536 + ndx;
+  const int zz = z + ndz, ff = f + ndt;
+  const bool zok = nlane && zz >= 0 && zz < P.N_z && ff >= 0 && ff < P.N_time;
+  const int zc = min(max(zz, 0), P.N_z - 1), fc = min(max(ff, 0), P.N_time - 1);
+  const int nplane = (fc * P.N_z + zc) * Nxy;
+  const int cbase = cell * Nxy;
+  const float two_c = 2.0f * nc;
+
+  int apix[G], anid[G], bpix[G], bnid[G];
+  float anb[G], bnb[G], bnv[G];
+  // packed pixels of the pending voxels, one per lane (A in lanes 0..G-1,
+  // B in lanes G..2G-1): one vector test per update finds the rare case of
+  // an in-plane neighbour among them
+  int pend = kNoPix;
+  auto fill = [&](int k, int pix) {
+    bpix[k] = pix;
+    const bool valid = pix != kNoPix;
+    const int yy = valid ? (pix >> 16) : 0, xx = valid ? (pix & 0xffff) : 0;
+    const int jj = yy + ndy, kk = xx + ndx;
+    const bool nok = valid & zok & (jj >= 0) & (jj < P.N_y) & (kk >= 0) & (kk < P.N_x);
+    bnb[k] = P.obj[(unsigned)(nok ? nplane + jj * P.N_x + kk : P.nobj + 64)];
+    bnid[k] = (valid & inplane) ? pix + noff : -2;
+    bnv[k] = __int_as_float(0x7fc00000);
+  };
+  double voxmag = 0.0;
+  unsigned long long nupd = 0, nzero = 0;
+  const int ngroups = (cnt + G - 1) / G;
+  int raw = list[n0 + min(lane, cnt - 1)];
+  wait_vm_all();
+  int pnext[G];
+#pragma unroll
+  for (int k = 0; k < G; k++) pnext[k] = k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+#pragma unroll
+  for (int k = 0; k < G; k++) {
+    apix[k] = kNoPix;
+    anid[k] = -2;
+    anb[k] = __int_as_float(0x7fc00000);
+  }
+  for (int g = 0; g <= ngroups; g++) {
+#pragma unroll
+    for (int k = 0; k < G; k++) fill(k, pnext[k]);
+    raw = list[n0 + min((g + 1) * G + lane, cnt - 1)];
+#pragma unroll
+    for (int k = 0; k < G; k++) pend = lane == G + k ? bpix[k] : pend;
+    float rx = 0.0f, rm = 0.0f;
+    int ridx = P.nobj + lane;
+#pragma unroll
+    for (int s = 0; s < G; s++) {
+      const int pix = apix[s];
+      if (pix == kNoPix) continue;  // only past the end of the sequence
+      const int n = (g - 1) * G + s;
+      const int par = n & 1;
+      const float nbv = anb[s];
+      const bool vld = (nbv == nbv) & (lane != 13);
+      const float V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
+      const bool anypos = __ballot((lane != 13) & (nbv > 0.0f)) != 0ull;
+      const float dlt = vld ? V - nbv : 0.0f;
+      const float t1 =
+          __builtin_amdgcn_exp2f(0.8f * __builtin_amdgcn_logf(fabsf(dlt))) * nisqp;
+      const float t3 = nc + t1;
+      const float b = fmaf(1.2f, t1, two_c) * __builtin_amdgcn_rcpf(t3 * t3 * nsq);
+      const float gb = (vld ? ng : 0.0f) * b;
+      float pb = gb, pbx = -gb * dlt;
+      wave_total_ff(pb, pbx);
+      const bool skip = (V == 0.0f) && iter > 1 && !anypos;
+      if (lane == 0) {
+        S.pr[par][0] = pb;
+        S.pr[par][1] = pbx;
+        S.pr[par][2] = V;
+        S.pr[par][3] = skip ? 1.0f : 0.0f;
+      }
+      __syncthreads();
+      float Vs;
+      bool sk;
+      const float xn = solve_step<MAXTH>(P, S, par, nth, &Vs, &sk);
+      const float dx = xn - V;
+      // (dy + 1) * 65536 + (dx + 1) with |dy|, |dx| <= 1 for a neighbour
+      const unsigned dd = (unsigned)(pend - pix + 65537);
+      const bool adj = (pend != kNoPix) & (dd != 65537u) & ((dd & 0xffffu) <= 2u) &
+                       (dd <= 131074u);
+      if (__ballot(adj) != 0ull) {
+#pragma unroll
+        for (int o = s + 1; o < G; o++) anb[o] = anid[o] == pix ? xn : anb[o];
+#pragma unroll
+        for (int o = 0; o < G; o++) bnv[o] = bnid[o] == pix ? xn : bnv[o];
+      }
+      const int lin = cbase + (pix >> 16) * P.N_x + (pix & 0xffff);
+      rx = lane == s ? xn : rx;
+      rm = lane == s ? (skip ? 0.0f : fabsf(dx)) : rm;
+      ridx = lane == s ? lin : ridx;
+      voxmag += (count_mag && !skip) ? fabs((double)xn) : 0.0;
+      nupd += skip ? 0 : 1;
+      nzero += skip ? 1 : 0;
+    }
+    wait_vm_all();
+    P.obj[(unsigned)ridx] = rx;
+    P.mag[(unsigned)ridx] = rm;
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      apix[k] = bpix[k];
+      anid[k] = bnid[k];
+      anb[k] = bnv[k] == bnv[k] ? bnv[k] : bnb[k];
+    }
+    // B moves to lanes 0..G-1
+    pend = __shfl(pend, (lane + G) & 63);
+    pend = lane < G ? pend : kNoPix;
+#pragma unroll
+    for (int k = 0; k < G; k++)
+      pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+  }
+  if (lane == 0 && seq != 2) {
+    P.cell_voxmag[cell] += voxmag;
+    P.cell_upd[cell] += nupd;
+    P.cell_zero[cell] += nzero;
+  }
+}
 
 template <int NB, int MZ, int MAXTH>
 __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ cells, int iter,
@@ -545,7 +694,7 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
     // The next voxel's (e, w) is read before this voxel's e is written and
     // corrected afterwards by a(this voxel, that bin) * dx: the same fma the
     // lane owning that entry stores, so no bits change.
-    constexpr int REC = NB == 5 ? 8 : 4;
+    constexpr int REC = col_rec(NB);
     const int lq = lane / EPV, ent = lane - lq * EPV;
     const int ub = ent / MZ, mm = ent - ub * MZ;
     const int q = wave * vpw + lq;
@@ -681,138 +830,7 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
     if (lane == 0) S.nent[wave] = (unsigned long long)ne;
   } else {
     // ---------------- prior wave ----------------
-    int ndz = 0, ndy = 0, ndx = 0, ndt = 0;
-    float ng = 0.0f, nsq = 1.0f, nisqp = 1.0f, nc = 1.0f;
-    const bool nlane = lane < 29;
-    if (nlane) {
-      if (lane < 27) {
-        ndz = lane / 9 - 1;
-        ndy = (lane / 3) % 3 - 1;
-        ndx = lane % 3 - 1;
-        ng = P.sfilt[lane];
-        nsq = P.sSq, nisqp = 1.0f / P.sSqp, nc = P.c_s;
-      } else {
-        ndt = lane == 27 ? -1 : 1;
-        ng = P.tfilt;
-        nsq = P.sTq, nisqp = 1.0f / P.sTqp, nc = P.c_t;
-      }
-    }
-    const bool inplane = lane < 27 && ndz == 0 && lane != 13;
-    const int noff = ndy * 65536 + ndx;
-    const int zz = z + ndz, ff = f + ndt;
-    const bool zok = nlane && zz >= 0 && zz < P.N_z && ff >= 0 && ff < P.N_time;
-    const int zc = min(max(zz, 0), P.N_z - 1), fc = min(max(ff, 0), P.N_time - 1);
-    const int nplane = (fc * P.N_z + zc) * Nxy;
-    const int cbase = cell * Nxy;
-    const float two_c = 2.0f * nc;
-
-    int apix[G], anid[G], bpix[G], bnid[G];
-    float anb[G], bnb[G], bnv[G];
-    // packed pixels of the pending voxels, one per lane (A in lanes 0..G-1,
-    // B in lanes G..2G-1): one vector test per update finds the rare case of
-    // an in-plane neighbour among them
-    int pend = kNoPix;
-    auto fill = [&](int k, int pix) {
-      bpix[k] = pix;
-      const bool valid = pix != kNoPix;
-      const int yy = valid ? (pix >> 16) : 0, xx = valid ? (pix & 0xffff) : 0;
-      const int jj = yy + ndy, kk = xx + ndx;
-      const bool nok = valid & zok & (jj >= 0) & (jj < P.N_y) & (kk >= 0) & (kk < P.N_x);
-      bnb[k] = P.obj[(unsigned)(nok ? nplane + jj * P.N_x + kk : P.nobj + 64)];
-      bnid[k] = (valid & inplane) ? pix + noff : -2;
-      bnv[k] = __int_as_float(0x7fc00000);
-    };
-    double voxmag = 0.0;
-    unsigned long long nupd = 0, nzero = 0;
-    const int ngroups = (cnt + G - 1) / G;
-    int raw = list[n0 + min(lane, cnt - 1)];
-    wait_vm_all();
-    int pnext[G];
-#pragma unroll
-    for (int k = 0; k < G; k++) pnext[k] = k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
-#pragma unroll
-    for (int k = 0; k < G; k++) {
-      apix[k] = kNoPix;
-      anid[k] = -2;
-      anb[k] = __int_as_float(0x7fc00000);
-    }
-    for (int g = 0; g <= ngroups; g++) {
-#pragma unroll
-      for (int k = 0; k < G; k++) fill(k, pnext[k]);
-      raw = list[n0 + min((g + 1) * G + lane, cnt - 1)];
-#pragma unroll
-      for (int k = 0; k < G; k++) pend = lane == G + k ? bpix[k] : pend;
-      float rx = 0.0f, rm = 0.0f;
-      int ridx = P.nobj + lane;
-#pragma unroll
-      for (int s = 0; s < G; s++) {
-        const int pix = apix[s];
-        if (pix == kNoPix) continue;  // only past the end of the sequence
-        const int n = (g - 1) * G + s;
-        const int par = n & 1;
-        const float nbv = anb[s];
-        const bool vld = (nbv == nbv) & (lane != 13);
-        const float V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
-        const bool anypos = __ballot((lane != 13) & (nbv > 0.0f)) != 0ull;
-        const float dlt = vld ? V - nbv : 0.0f;
-        const float t1 =
-            __builtin_amdgcn_exp2f(0.8f * __builtin_amdgcn_logf(fabsf(dlt))) * nisqp;
-        const float t3 = nc + t1;
-        const float b = fmaf(1.2f, t1, two_c) * __builtin_amdgcn_rcpf(t3 * t3 * nsq);
-        const float gb = (vld ? ng : 0.0f) * b;
-        float pb = gb, pbx = -gb * dlt;
-        wave_total_ff(pb, pbx);
-        const bool skip = (V == 0.0f) && iter > 1 && !anypos;
-        if (lane == 0) {
-          S.pr[par][0] = pb;
-          S.pr[par][1] = pbx;
-          S.pr[par][2] = V;
-          S.pr[par][3] = skip ? 1.0f : 0.0f;
-        }
-        __syncthreads();
-        float Vs;
-        bool sk;
-        const float xn = solve_step<MAXTH>(P, S, par, nth, &Vs, &sk);
-        const float dx = xn - V;
-        // (dy + 1) * 65536 + (dx + 1) with |dy|, |dx| <= 1 for a neighbour
-        const unsigned dd = (unsigned)(pend - pix + 65537);
-        const bool adj = (pend != kNoPix) & (dd != 65537u) & ((dd & 0xffffu) <= 2u) &
-                         (dd <= 131074u);
-        if (__ballot(adj) != 0ull) {
-#pragma unroll
-          for (int o = s + 1; o < G; o++) anb[o] = anid[o] == pix ? xn : anb[o];
-#pragma unroll
-          for (int o = 0; o < G; o++) bnv[o] = bnid[o] == pix ? xn : bnv[o];
-        }
-        const int lin = cbase + (pix >> 16) * P.N_x + (pix & 0xffff);
-        rx = lane == s ? xn : rx;
-        rm = lane == s ? (skip ? 0.0f : fabsf(dx)) : rm;
-        ridx = lane == s ? lin : ridx;
-        voxmag += (count_mag && !skip) ? fabs((double)xn) : 0.0;
-        nupd += skip ? 0 : 1;
-        nzero += skip ? 1 : 0;
-      }
-      wait_vm_all();
-      P.obj[(unsigned)ridx] = rx;
-      P.mag[(unsigned)ridx] = rm;
-#pragma unroll
-      for (int k = 0; k < G; k++) {
-        apix[k] = bpix[k];
-        anid[k] = bnid[k];
-        anb[k] = bnv[k] == bnv[k] ? bnv[k] : bnb[k];
-      }
-      // B moves to lanes 0..G-1
-      pend = __shfl(pend, (lane + G) & 63);
-      pend = lane < G ? pend : kNoPix;
-#pragma unroll
-      for (int k = 0; k < G; k++)
-        pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
-    }
-    if (lane == 0 && seq != 2) {
-      P.cell_voxmag[cell] += voxmag;
-      P.cell_upd[cell] += nupd;
-      P.cell_zero[cell] += nzero;
-    }
+    prior_wave<MAXTH>(P, S, list, n0, cnt, f, z, iter, seq, count_mag, nth, lane);
   }
 
   // e back to global memory; entry count
@@ -833,6 +851,255 @@ template __global__ void k_icd_mw<3, 1, 8>(KP, const int2 *, int, int, int, int,
 template __global__ void k_icd_mw<3, 1, 15>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mw<5, 2, 8>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mw<5, 2, 15>(KP, const int2 *, int, int, int, int, int);
+
+// ---------------------------------------------------------------------------
+// Multi-wavefront voxel update with the error sinogram left in global memory
+// (L2-resident), for cells whose [views][N_r][rows] block does not fit in
+// LDS: the coarse multires stages (5 or 9 bins x 2 or 4 rows per view) and
+// large finest stages.  The roles are those of k_icd_mw: theta waves own
+// whole views of the cell and are the only waves that read or write their e
+// rows, so the sequential sweep follows from program order (one wavefront's
+// global loads and stores to one address take effect in issue order); the
+// prior wave is the same.  A lane holds one (view, bin) of the voxel's column
+// with all MZ detector rows of the slice, which are contiguous in the
+// [N_z][N_p][N_r][mz] layout, for VPL views.  Per voxel n a lane:
+//   issues the column-table entry (a, w0) of voxel n + 2 and the (e, w) rows
+//   of voxel n + 1 (addressed by the w0 that arrived during voxel n - 1),
+//   reduces voxel n's theta sums, meets the other waves, solves,
+//   waits once for everything it issued, stores voxel n's e, and corrects
+//   voxel n + 1's e, read before that store, by a(voxel n, that bin) * dx:
+//   the same fma the lane owning the bin stores, so no bits change.
+// ---------------------------------------------------------------------------
+template <int MZ>
+__device__ __forceinline__ void ld_rows(const float *p, float (&o)[MZ]) {
+  if constexpr (MZ == 1) {
+    o[0] = p[0];
+  } else if constexpr (MZ == 2) {
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < MZ; i += 4) {
+      const float4 v = *reinterpret_cast<const float4 *>(p + i);
+      o[i] = v.x;
+      o[i + 1] = v.y;
+      o[i + 2] = v.z;
+      o[i + 3] = v.w;
+    }
+  }
+}
+
+template <int MZ>
+__device__ __forceinline__ void st_rows(float *p, const float (&o)[MZ]) {
+  if constexpr (MZ == 1) {
+    p[0] = o[0];
+  } else if constexpr (MZ == 2) {
+    *reinterpret_cast<float2 *>(p) = make_float2(o[0], o[1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < MZ; i += 4)
+      *reinterpret_cast<float4 *>(p + i) = make_float4(o[i], o[i + 1], o[i + 2], o[i + 3]);
+  }
+}
+
+template <int NB, int MZ, int VPL, int MAXTH>
+__global__ __launch_bounds__(1024) void k_icd_mwg(KP P, const int2 *__restrict__ cells, int iter,
+                                                  int chunk, int seq, int count_mag, int vpw) {
+  constexpr int G = 8;             // pixels read per list load
+  constexpr int VW = 64 / NB;      // views of one lane layer
+  constexpr int REC = col_rec(NB);
+  __shared__ StepIn S;
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int Nxy = P.N_x * P.N_y;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0) return;
+  if (threadIdx.x < 32) {
+    (&S.t1[0][0])[threadIdx.x] = 0.0;
+    (&S.t2[0][0])[threadIdx.x] = 0.0f;
+  }
+  __syncthreads();
+
+  if (wave < nth) {
+    // ---------------- theta wave ----------------
+    const int lq = lane / NB, ub = lane - lq * NB;
+    bool act[VPL];
+    int qv[VPL];
+    size_t ebase[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; k++) {
+      const int qq = k * VW + lq, q = wave * vpw + qq;
+      act[k] = (lq < VW) & (qq < vpw) & (q < nv);
+      qv[k] = act[k] ? q : 0;
+      ebase[k] = sidx(P, z, nv > 0 ? P.view_idx[vb + qv[k]] : 0, ub, 0);
+    }
+    const float *colt = P.colt;
+    const size_t fbase = (size_t)f * Nxy;
+    auto rec_at = [&](int pix, int k) -> size_t {
+      const int lin = pix != kNoPix ? (pix >> 16) * P.N_x + (pix & 0xffff) : 0;
+      return ((fbase + lin) * (size_t)P.colt_nv + qv[k]) * REC;
+    };
+    const int *lst = list + n0;
+    auto pix_at = [&](int raw, int k, int n) -> int {
+      return n < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+    };
+
+    // voxel n ("c"): a, w0, (e corrected, w) rows and their address;
+    // voxel n + 1 ("n"): a, w0 arrived, (e, w) rows in flight
+    float ca[VPL], na[VPL], ce[VPL][MZ], cwv[VPL][MZ], ne[VPL][MZ], nwv[VPL][MZ];
+    int cw0[VPL], nw0[VPL];
+    size_t cea[VPL], nea[VPL];
+    int rawc = lst[min(lane, cnt - 1)];
+    int rawn = lst[min(G + lane, cnt - 1)];
+    wait_vm_all();
+    {
+      const int p0 = pix_at(rawc, 0, 0), p1 = pix_at(rawc, 1, 1);
+      float a0[VPL], a1[VPL];
+      int w00[VPL], w01[VPL];
+#pragma unroll
+      for (int k = 0; k < VPL; k++) {
+        const size_t r0 = rec_at(p0, k), r1 = rec_at(p1, k);
+        a0[k] = colt[r0 + ub];
+        w00[k] = __float_as_int(colt[r0 + NB]);
+        a1[k] = colt[r1 + ub];
+        w01[k] = __float_as_int(colt[r1 + NB]);
+      }
+      wait_vm_all();
+#pragma unroll
+      for (int k = 0; k < VPL; k++) {
+        cw0[k] = act[k] ? w00[k] : -1;
+        ca[k] = cw0[k] >= 0 ? a0[k] : 0.0f;
+        const bool v1 = act[k] & (p1 != kNoPix);
+        nw0[k] = v1 ? w01[k] : -1;
+        na[k] = nw0[k] >= 0 ? a1[k] : 0.0f;
+        cea[k] = ebase[k] + (size_t)max(cw0[k], 0) * MZ;
+        ld_rows<MZ>(P.e + cea[k], ce[k]);
+        ld_rows<MZ>(P.w + cea[k], cwv[k]);
+      }
+      wait_vm_all();
+    }
+    int rawnn = rawn;
+    unsigned long long nent = 0;
+    const int ngroups = (cnt + G - 1) / G;
+    for (int g = 0; g < ngroups; g++) {
+#pragma unroll
+      for (int s = 0; s < G; s++) {
+        const int n = g * G + s;
+        if (n >= cnt) break;
+        const int par = n & 1;
+        if (s == 0) rawnn = lst[min((g + 2) * G + lane, cnt - 1)];
+        // issue: column entries of voxel n + 2, (e, w) rows of voxel n + 1
+        const int p2 = s + 2 < G ? pix_at(rawc, s + 2, n + 2) : pix_at(rawn, s + 2 - G, n + 2);
+        float a2[VPL];
+        int w2[VPL];
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          const size_t r = rec_at(p2, k);
+          a2[k] = colt[r + ub];
+          w2[k] = __float_as_int(colt[r + NB]);
+          nea[k] = ebase[k] + (size_t)max(nw0[k], 0) * MZ;
+          ld_rows<MZ>(P.e + nea[k], ne[k]);
+          ld_rows<MZ>(P.w + nea[k], nwv[k]);
+        }
+        // voxel n's theta sums (XT_VoxUpdate.c:61-92)
+        double t1 = 0.0;
+        float t2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+#pragma unroll
+          for (int m = 0; m < MZ; m++) {
+            const float e = ce[k][m], w = cwv[k][m];
+            const float cw_out =
+                P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
+            const float cw = huber_sel(P, e, w) ? w : cw_out;
+            const float ac = ca[k] * cw;
+            t1 += (double)(-e * ac);
+            t2 += ca[k] * ac;
+          }
+          nent += ca[k] > 0.0f ? MZ : 0;
+        }
+        wave_total_df(t1, t2);
+        if (lane == 0) {
+          S.t1[par][wave] = t1;
+          S.t2[par][wave] = t2;
+        }
+        // a of this voxel at the bin each lane holds for voxel n + 1
+        float cn[VPL];
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          const int d = nw0[k] + ub - cw0[k];
+          const bool ov = (nw0[k] >= 0) & (cw0[k] >= 0) & (d >= 0) & (d < NB);
+          const float av = __shfl(ca[k], lq * NB + (ov ? d : 0));
+          cn[k] = ov ? av : 0.0f;
+        }
+        __syncthreads();
+        float V;
+        bool skip;
+        const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
+        const float dx = xn - V;
+        wait_vm_all();
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          if (ca[k] != 0.0f) {
+            float o[MZ];
+#pragma unroll
+            for (int m = 0; m < MZ; m++) o[m] = fmaf(-ca[k], dx, ce[k][m]);
+            st_rows<MZ>(P.e + cea[k], o);
+          }
+#pragma unroll
+          for (int m = 0; m < MZ; m++) {
+            ce[k][m] = fmaf(-cn[k], dx, ne[k][m]);
+            cwv[k][m] = nwv[k][m];
+          }
+          ca[k] = na[k];
+          cw0[k] = nw0[k];
+          cea[k] = nea[k];
+          const bool v2 = act[k] & (p2 != kNoPix);
+          nw0[k] = v2 ? w2[k] : -1;
+          na[k] = nw0[k] >= 0 ? a2[k] : 0.0f;
+        }
+      }
+      rawc = rawn;
+      rawn = rawnn;
+    }
+    const double nsum = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
+  } else {
+    // ---------------- prior wave ----------------
+    prior_wave<MAXTH>(P, S, list, n0, cnt, f, z, iter, seq, count_mag, nth, lane);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    P.cell_ent[cell] += ne;
+  }
+}
+
+template __global__ void k_icd_mwg<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 4, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 4, 2, 15>(KP, const int2 *, int, int, int, int, int);
 
 // finest stage (m_xy = m_z = 1: at most 3 bins per view, one detector row)
 template __global__ void k_icd_lds<1, 3, 1, 4>(KP, const int2 *, int, int, int, int);
