@@ -116,58 +116,63 @@ __device__ __forceinline__ float qggmrf_b(float d, float sq, float sqp, float c)
   return ((float)kMrfQ - (float)(kMrfQ - kMrfP) * t1 / t3) / (t3 * sq);
 }
 
-// Wavefront sums over DPP lane moves (quad_perm, row mirrors, and the gfx9
-// row broadcasts 15/31); the total lands in lane 63 and is read back into a
-// scalar register.  Every lane must be active.  Moves that write every lane
-// need no initial value; the two broadcasts write only rows 1/3 and 2/3, the
-// other rows keep the 0 they start from.
+// Wavefront sums: four DPP levels inside each row of 16 lanes (quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror) leave every
+// lane holding its row's sum; then gfx950's permlane16 / permlane32 swaps
+// pair rows (0,1), (2,3) and then the two halves.  A swap of a register
+// with itself leaves one copy with one row of the pair and the other copy
+// with the other row, so their sum is the pair's total, in every lane.  The
+// total ends in every lane ((r0 + r1) + (r2 + r3)).  Every lane must be
+// active.
 template <int CTRL>
 __device__ __forceinline__ int dpp_all(int v) {
   return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
-template <int CTRL, int RM>
-__device__ __forceinline__ int dpp_rows(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, RM, 0xF, false);
-}
-template <int CTRL, int RM = 0xF>
+template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
-  return __int_as_float(RM == 0xF ? dpp_all<CTRL>(__float_as_int(v))
-                                  : dpp_rows<CTRL, RM>(__float_as_int(v)));
+  return __int_as_float(dpp_all<CTRL>(__float_as_int(v)));
 }
-template <int CTRL, int RM = 0xF>
+template <int CTRL>
 __device__ __forceinline__ double dpp_d(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const int l = (int)(unsigned)b, h = (int)(unsigned)(b >> 32);
-  const unsigned lo = (unsigned)(RM == 0xF ? dpp_all<CTRL>(l) : dpp_rows<CTRL, RM>(l));
-  const unsigned hi = (unsigned)(RM == 0xF ? dpp_all<CTRL>(h) : dpp_rows<CTRL, RM>(h));
+  const unsigned lo = (unsigned)dpp_all<CTRL>((int)(unsigned)b);
+  const unsigned hi = (unsigned)dpp_all<CTRL>((int)(unsigned)(b >> 32));
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-__device__ __forceinline__ float lane63_f(float v) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+template <bool HALVES>
+__device__ __forceinline__ float swap_sum_f(float v) {
+  const unsigned u = __float_as_uint(v);
+  const auto r = HALVES ? __builtin_amdgcn_permlane32_swap(u, u, false, false)
+                        : __builtin_amdgcn_permlane16_swap(u, u, false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
 }
-__device__ __forceinline__ double lane63_d(double v) {
+template <bool HALVES>
+__device__ __forceinline__ double swap_sum_d(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  const unsigned l = (unsigned)b, h = (unsigned)(b >> 32);
+  const auto lo = HALVES ? __builtin_amdgcn_permlane32_swap(l, l, false, false)
+                         : __builtin_amdgcn_permlane16_swap(l, l, false, false);
+  const auto hi = HALVES ? __builtin_amdgcn_permlane32_swap(h, h, false, false)
+                         : __builtin_amdgcn_permlane16_swap(h, h, false, false);
+  const double a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  const double c = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+  return a + c;
 }
 __device__ __forceinline__ float wave_total(float v) {
-  v += dpp_f<0xB1>(v);        // quad_perm [1,0,3,2]
-  v += dpp_f<0x4E>(v);        // quad_perm [2,3,0,1]
-  v += dpp_f<0x141>(v);       // row_half_mirror
-  v += dpp_f<0x140>(v);       // row_mirror: row sums in every lane of a row
-  v += dpp_f<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_f<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-  return lane63_f(v);
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  v += dpp_f<0x140>(v);  // row_mirror: row sums in every lane of a row
+  v = swap_sum_f<false>(v);
+  return swap_sum_f<true>(v);
 }
 __device__ __forceinline__ double wave_total_d(double v) {
   v += dpp_d<0xB1>(v);
   v += dpp_d<0x4E>(v);
   v += dpp_d<0x141>(v);
   v += dpp_d<0x140>(v);
-  v += dpp_d<0x142, 0xA>(v);
-  v += dpp_d<0x143, 0xC>(v);
-  return lane63_d(v);
+  v = swap_sum_d<false>(v);
+  return swap_sum_d<true>(v);
 }
 // A double and a float sum taken level by level, so that each level's lane
 // moves fill the other tree's wait states.
@@ -180,15 +185,14 @@ __device__ __forceinline__ void wave_total_df(double &d, float &f) {
   f += dpp_f<0x141>(f);
   d += dpp_d<0x140>(d);
   f += dpp_f<0x140>(f);
-  d += dpp_d<0x142, 0xA>(d);
-  f += dpp_f<0x142, 0xA>(f);
-  d += dpp_d<0x143, 0xC>(d);
-  f += dpp_f<0x143, 0xC>(f);
-  d = lane63_d(d);
-  f = lane63_f(f);
+  d = swap_sum_d<false>(d);
+  f = swap_sum_f<false>(f);
+  d = swap_sum_d<true>(d);
+  f = swap_sum_f<true>(f);
 }
-// Two float sums taken level by level.
-__device__ __forceinline__ void wave_total_ff(float &a, float &b) {
+// Two float sums over lanes 0..31 only (rows 0 and 1), taken level by level;
+// the totals end in every lane of those rows.
+__device__ __forceinline__ void lo32_total_ff(float &a, float &b) {
   a += dpp_f<0xB1>(a);
   b += dpp_f<0xB1>(b);
   a += dpp_f<0x4E>(a);
@@ -197,12 +201,8 @@ __device__ __forceinline__ void wave_total_ff(float &a, float &b) {
   b += dpp_f<0x141>(b);
   a += dpp_f<0x140>(a);
   b += dpp_f<0x140>(b);
-  a += dpp_f<0x142, 0xA>(a);
-  b += dpp_f<0x142, 0xA>(b);
-  a += dpp_f<0x143, 0xC>(a);
-  b += dpp_f<0x143, 0xC>(b);
-  a = lane63_f(a);
-  b = lane63_f(b);
+  a = swap_sum_f<false>(a);
+  b = swap_sum_f<false>(b);
 }
 
 }  // namespace tb

@@ -542,7 +542,7 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
     bnid[k] = (valid & inplane) ? pix + noff : -2;
     bnv[k] = __int_as_float(0x7fc00000);
   };
-  double voxmag = 0.0;
+  double vmacc = 0.0;  // lane k: magnitudes of the voxels at group position k
   unsigned long long nupd = 0, nzero = 0;
   const int ngroups = (cnt + G - 1) / G;
   int raw = list[n0 + min(lane, cnt - 1)];
@@ -564,6 +564,7 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
     for (int k = 0; k < G; k++) pend = lane == G + k ? bpix[k] : pend;
     float rx = 0.0f, rm = 0.0f;
     int ridx = P.nobj + lane;
+    int rsk = -1;  // lane k: voxel k of the group skipped (1), updated (0), absent (-1)
 #pragma unroll
     for (int s = 0; s < G; s++) {
       const int pix = apix[s];
@@ -581,7 +582,7 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
       const float b = fmaf(1.2f, t1, two_c) * __builtin_amdgcn_rcpf(t3 * t3 * nsq);
       const float gb = (vld ? ng : 0.0f) * b;
       float pb = gb, pbx = -gb * dlt;
-      wave_total_ff(pb, pbx);
+      lo32_total_ff(pb, pbx);  // neighbour lanes are 0..28
       const bool skip = (V == 0.0f) && iter > 1 && !anypos;
       if (lane == 0) {
         S.pr[par][0] = pb;
@@ -608,10 +609,13 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
       rx = lane == s ? xn : rx;
       rm = lane == s ? (skip ? 0.0f : fabsf(dx)) : rm;
       ridx = lane == s ? lin : ridx;
-      voxmag += (count_mag && !skip) ? fabs((double)xn) : 0.0;
-      nupd += skip ? 0 : 1;
-      nzero += skip ? 1 : 0;
+      rsk = lane == s ? (skip ? 1 : 0) : rsk;
     }
+    // the group's update counts and |x| sums (updateVoxels_Atten's return
+    // value, XT_VoxUpdate.c:263-281), lane by lane
+    vmacc += (count_mag && rsk == 0) ? fabs((double)rx) : 0.0;
+    nupd += (unsigned long long)__popcll(__ballot(rsk == 0));
+    nzero += (unsigned long long)__popcll(__ballot(rsk == 1));
     wait_vm_all();
     P.obj[(unsigned)ridx] = rx;
     P.mag[(unsigned)ridx] = rm;
@@ -628,6 +632,7 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
     for (int k = 0; k < G; k++)
       pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
   }
+  const double voxmag = wave_sum_d(vmacc);
   if (lane == 0 && seq != 2) {
     P.cell_voxmag[cell] += voxmag;
     P.cell_upd[cell] += nupd;
