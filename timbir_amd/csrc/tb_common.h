@@ -76,12 +76,12 @@ inline size_t icd_lds_bytes(int nv, int N_r, int mz, int nb) {
   return ew * 2 * sizeof(float) + (size_t)nv * kHrStride * sizeof(float);
 }
 
-// Dynamic LDS bytes of the multi-wavefront kernel for a cell of nv views and
-// nth theta waves: the (e, w) block, one dummy word pair per theta-wave lane,
-// the views' detector-response rows.
-inline size_t icd_mw_lds_bytes(int nv, int N_r, int mz, int nth) {
-  const size_t ew = (size_t)nv * (N_r * mz + 1) + (size_t)nth * 64;
-  return ew * 2 * sizeof(float) + (size_t)nv * kHrStride * sizeof(float);
+// Dynamic LDS bytes of the multi-wavefront kernel for a cell of nv views
+// and nlanes theta-wave lane slots (theta waves x views per lane x 64): the
+// (e, w) block with an odd row stride and one dummy word pair per lane slot.
+inline size_t icd_mw_lds_bytes(int nv, int N_r, int mz, int nlanes) {
+  const size_t ew = (size_t)nv * (N_r * mz + 1) + (size_t)nlanes;
+  return ew * 2 * sizeof(float);
 }
 
 // Colour-schedule voxel order: keyed 4-round Feistel bijection on

@@ -22,7 +22,7 @@ template <int NT>
 __global__ void k_update_voxels(KP, const int2 *, int, int, int, int, int);
 template <int VPL, int NB, int MZ, int G>
 __global__ void k_icd_lds(KP, const int2 *, int, int, int, int);
-template <int NB, int MZ, int MAXTH>
+template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mw(KP, const int2 *, int, int, int, int, int);
 template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
@@ -851,14 +851,26 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     return 0;
   };
   if (nb && (g_opt.kernel == 0 || g_opt.kernel == 3)) {
-    const int epv = nb * st->mult_z;
-    const int nth = (st->max_nv + 64 / epv - 1) / (64 / epv);
-    const int vpw = (st->max_nv + nth - 1) / nth;
+    // one wavefront per SIMD where the views allow: at most 3 theta waves
+    // (one view per lane, else two), else one view per lane and more waves
+    const int vw = 64 / (nb * st->mult_z);
+    const int nv = st->max_nv;
+    int vpl = 1, nth = (nv + vw - 1) / vw;
+    if (nth > 3 && (nv + 2 * vw - 1) / (2 * vw) <= 3) {
+      vpl = 2;
+      nth = (nv + 2 * vw - 1) / (2 * vw);
+    }
+    const int vpw = (nv + nth - 1) / nth;
     mw_fn fn = nullptr;
-    if (nb == 3) fn = nth <= 4 ? k_icd_mw<3, 1, 4> : nth <= 8 ? k_icd_mw<3, 1, 8>
-                    : nth <= 15 ? k_icd_mw<3, 1, 15> : nullptr;
-    else fn = nth <= 8 ? k_icd_mw<5, 2, 8> : nth <= 15 ? k_icd_mw<5, 2, 15> : nullptr;
-    const size_t lds = icd_mw_lds_bytes(st->max_nv, st->N_r, st->mult_z, nth);
+    if (nb == 3) {
+      if (vpl == 2) fn = nth <= 2 ? k_icd_mw<3, 1, 2, 2> : k_icd_mw<3, 1, 2, 3>;
+      else fn = nth <= 4 ? k_icd_mw<3, 1, 1, 4> : nth <= 8 ? k_icd_mw<3, 1, 1, 8>
+              : nth <= 15 ? k_icd_mw<3, 1, 1, 15> : nullptr;
+    } else {
+      if (vpl == 2) fn = k_icd_mw<5, 2, 2, 3>;
+      else fn = nth <= 8 ? k_icd_mw<5, 2, 1, 8> : nth <= 15 ? k_icd_mw<5, 2, 1, 15> : nullptr;
+    }
+    const size_t lds = icd_mw_lds_bytes(nv, st->N_r, st->mult_z, nth * vpl * 64);
     if (fn && lds + 1024 <= st->lds_max && run_mw(fn, nb, nth, vpw, lds, false) < 0) return -1;
   }
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 4)) {

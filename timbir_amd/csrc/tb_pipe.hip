@@ -397,22 +397,23 @@ __device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int pa
                                             float *V_out, bool *skip_out) {
 #pragma clang fp contract(off)
   // every slot is read at once (one LDS latency) and added as a fixed
-  // pairwise tree; slots of absent waves hold zero
-  constexpr int N2 = MAXTH <= 4 ? 4 : MAXTH <= 8 ? 8 : 16;
-  double a1[N2];
-  float a2[N2];
+  // pairwise tree over the MAXTH slots; slots of absent waves hold zero
+  double a1[MAXTH];
+  float a2[MAXTH];
 #pragma unroll
-  for (int w = 0; w < N2; w++) {
-    a1[w] = w < MAXTH ? S.t1[par][w] : 0.0;
-    a2[w] = w < MAXTH ? S.t2[par][w] : 0.0f;
+  for (int w = 0; w < MAXTH; w++) {
+    a1[w] = S.t1[par][w];
+    a2[w] = S.t2[par][w];
   }
 #pragma unroll
-  for (int h = N2 / 2; h >= 1; h /= 2)
+  for (int m = MAXTH; m > 1; m = (m + 1) / 2) {
+    const int h = (m + 1) / 2;
 #pragma unroll
-    for (int w = 0; w < h; w++) {
+    for (int w = 0; w + h < m; w++) {
       a1[w] = a1[w] + a1[w + h];
       a2[w] = a2[w] + a2[w + h];
     }
+  }
   const double T1 = a1[0];
   const float T2 = a2[0];
   (void)nth;
@@ -640,12 +641,15 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
   }
 }
 
-template <int NB, int MZ, int MAXTH>
-__global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ cells, int iter,
-                                                 int chunk, int seq, int count_mag, int vpw) {
+template <int NB, int MZ, int VPL, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mw(KP P, const int2 *__restrict__ cells,
+                                                            int iter, int chunk, int seq,
+                                                            int count_mag, int vpw) {
   constexpr int EPV = NB * MZ;  // entries of one view
-  constexpr int G = 8;          // prior wave: voxels per prefetch group
-  extern __shared__ float2 s_ew[];  // [nv][RS] (e, w), per-lane dummies, then H rows
+  constexpr int VW = 64 / EPV;  // views of one lane layer
+  constexpr int G = 8;          // voxels per prefetch group
+  constexpr int REC = col_rec(NB);
+  extern __shared__ float2 s_ew[];  // [nv][RS] (e, w), then one dummy pair per lane and layer
   __shared__ StepIn S;
   const int2 cl = cells[blockIdx.x];
   const int f = cl.x, z = cl.y;
@@ -657,7 +661,6 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
   const int RW = P.N_r * MZ;
   const int RS = RW + 1;
   const int rows = nv * RS;
-  float *s_h = reinterpret_cast<float *>(s_ew + rows + nth * 64);
 
   const int *list;
   int n0 = 0, cnt = 0;
@@ -674,18 +677,15 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
   }
   if (cnt <= 0) return;
 
-  // stage the (e, w) block (whole rows per wave) and the response rows
+  // stage the (e, w) block (whole rows per wave)
   const int nwaves = nth + 1;
   for (int q = wave; q < nv; q += nwaves) {
     const size_t g = sidx(P, z, P.view_idx[vb + q], 0, 0);
     for (int t = lane; t < RW; t += 64) s_ew[q * RS + t] = make_float2(P.e[g + t], P.w[g + t]);
     if (lane == 0) s_ew[q * RS + RW] = make_float2(0.0f, 0.0f);
   }
-  for (int t = threadIdx.x; t < nth * 64; t += blockDim.x) s_ew[rows + t] = make_float2(0.f, 0.f);
-  for (int i = threadIdx.x; i < nv * kHrStride; i += blockDim.x) {
-    const int q = i / kHrStride, b = i - q * kHrStride;
-    s_h[i] = P.hr[(size_t)P.view_idx[vb + q] * kHrStride + b];
-  }
+  for (int t = threadIdx.x; t < nth * VPL * 64; t += blockDim.x)
+    s_ew[rows + t] = make_float2(0.f, 0.f);
   if (threadIdx.x < 32) {
     (&S.t1[0][0])[threadIdx.x] = 0.0;
     (&S.t2[0][0])[threadIdx.x] = 0.0f;
@@ -694,28 +694,33 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
 
   if (wave < nth) {
     // ---------------- theta wave ----------------
-    // The lane's entry is (view q, bin w0 + ub, row mm); its A value and w0
-    // come from the column table, prefetched one group of G voxels ahead.
-    // The next voxel's (e, w) is read before this voxel's e is written and
-    // corrected afterwards by a(this voxel, that bin) * dx: the same fma the
-    // lane owning that entry stores, so no bits change.
-    constexpr int REC = col_rec(NB);
+    // Lane layer k holds the entry (view q_k, bin w0 + ub, row mm) of the
+    // current voxel; its A value and w0 come from the column table,
+    // prefetched one group of G voxels ahead.  The next voxel's (e, w) is
+    // read before this voxel's e is written and corrected afterwards by
+    // a(this voxel, that bin) * dx: the same fma the lane owning that entry
+    // stores, so no bits change.
     const int lq = lane / EPV, ent = lane - lq * EPV;
     const int ub = ent / MZ, mm = ent - ub * MZ;
-    const int q = wave * vpw + lq;
-    const bool active = lq < vpw && q < nv;
-    const int qv = active ? q : 0;
-    const int dummy = rows + wave * 64 + lane;
+    bool active[VPL];
+    int qv[VPL], dummy[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; k++) {
+      const int qq = k * VW + lq, q = wave * vpw + qq;
+      active[k] = (lq < VW) & (qq < vpw) & (q < nv);
+      qv[k] = active[k] ? q : 0;
+      dummy[k] = rows + (wave * VPL + k) * 64 + lane;
+    }
     const float4 *colt = reinterpret_cast<const float4 *>(P.colt);
     const size_t fbase = (size_t)f * Nxy;
     struct Rec {
       float a[NB];
       int w0;
     };
-    auto load = [&](int pix) -> Rec {
+    auto load = [&](int pix, int k) -> Rec {
       const bool valid = pix != kNoPix;
       const int lin = valid ? (pix >> 16) * P.N_x + (pix & 0xffff) : 0;
-      const size_t r = ((fbase + lin) * P.colt_nv + qv) * (REC / 4);
+      const size_t r = ((fbase + lin) * P.colt_nv + qv[k]) * (REC / 4);
       Rec o;
       const float4 r0 = colt[r];
       if (REC == 4) {
@@ -727,51 +732,66 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
         const float4 r1 = colt[r + 1];
         const float v[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 #pragma unroll
-        for (int k = 0; k < NB; k++) o.a[k] = v[k];
+        for (int t = 0; t < NB; t++) o.a[t] = v[t];
         o.w0 = __float_as_int(v[NB]);
       }
       // no arithmetic on the loaded registers here: any would make the
       // compiler wait for the load at once; validity is applied at use
       return o;
     };
-    auto pick = [&](const Rec &r, int k) -> float {
+    auto pick = [&](const Rec &r, int j) -> float {
       float v = 0.0f;
 #pragma unroll
-      for (int t = 0; t < NB; t++) v = k == t ? r.a[t] : v;
+      for (int t = 0; t < NB; t++) v = j == t ? r.a[t] : v;
       return v;
     };
     // w0 of a record as this lane uses it: -1 for an empty slot or a lane
     // without a view
-    auto w0_of = [&](const Rec &r, int pix) -> int {
-      return (active & (pix != kNoPix)) ? r.w0 : -1;
+    auto w0_of = [&](const Rec &r, int pix, int k) -> int {
+      return (active[k] & (pix != kNoPix)) ? r.w0 : -1;
     };
-    auto entry_at = [&](int w0) -> int { return w0 >= 0 ? qv * RS + (w0 + ub) * MZ + mm : dummy; };
+    auto entry_at = [&](int w0, int k) -> int {
+      return w0 >= 0 ? qv[k] * RS + (w0 + ub) * MZ + mm : dummy[k];
+    };
 
     int apix[G], bpix[G];
-    Rec arec[G], brec[G];
+    Rec arec[G][VPL], brec[G][VPL];
     const int ngroups = (cnt + G - 1) / G;
     int raw = list[n0 + min(lane, cnt - 1)];
     int pnext[G];
 #pragma unroll
-    for (int k = 0; k < G; k++) pnext[k] = k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+    for (int j = 0; j < G; j++) pnext[j] = j < cnt ? __builtin_amdgcn_readlane(raw, j) : kNoPix;
 #pragma unroll
-    for (int k = 0; k < G; k++) {
-      apix[k] = kNoPix;
-      arec[k].w0 = -1;
+    for (int j = 0; j < G; j++) {
+      apix[j] = kNoPix;
 #pragma unroll
-      for (int t = 0; t < NB; t++) arec[k].a[t] = 0.0f;
+      for (int k = 0; k < VPL; k++) {
+        arec[j][k].w0 = -1;
+#pragma unroll
+        for (int t = 0; t < NB; t++) arec[j][k].a[t] = 0.0f;
+      }
     }
-    // current voxel's entry, its (e, w) read ahead, and the correction
-    float ca = 0.0f, cprev = 0.0f, dxprev = 0.0f;
-    int cat = dummy, cw0 = -1;
-    float2 ews = make_float2(0.0f, 0.0f);
-    Rec crec = arec[0];
-    unsigned long long nent = 0;
+    // current voxel's entries: A value, (e, w) with e already corrected for
+    // the previous voxel's update, LDS slot, window start and record
+    float ca[VPL], ce[VPL], cwv[VPL];
+    int cat[VPL], cw0[VPL];
+    Rec crec[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; k++) {
+      ca[k] = 0.0f;
+      ce[k] = 0.0f;
+      cwv[k] = 0.0f;
+      cat[k] = dummy[k];
+      cw0[k] = -1;
+      crec[k] = arec[0][k];
+    }
+    unsigned nent = 0;  // entries with a > 0 (at most 2 x 65535 x 3 per lane)
     for (int g = 0; g <= ngroups; g++) {
 #pragma unroll
-      for (int k = 0; k < G; k++) {
-        bpix[k] = pnext[k];
-        brec[k] = load(pnext[k]);
+      for (int j = 0; j < G; j++) {
+        bpix[j] = pnext[j];
+#pragma unroll
+        for (int k = 0; k < VPL; k++) brec[j][k] = load(pnext[j], k);
       }
       raw = list[n0 + min((g + 1) * G + lane, cnt - 1)];
 #pragma unroll
@@ -780,55 +800,84 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
         if (pix == kNoPix) continue;  // only past the end of the sequence
         const int n = (g - 1) * G + s;
         const int par = n & 1;
-        const float e = fmaf(-cprev, dxprev, ews.x), w = ews.y;
-        const float cw_out =
-            P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
-        const float cw = huber_sel(P, e, w) ? w : cw_out;
-        const float ac = ca * cw;
-        double t1 = (double)(-e * ac);
-        float t2 = ca * ac;
+        // this voxel's partial sums (XT_VoxUpdate.c:61-92)
+        double t1;
+        float t2;
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          const float w = cwv[k];
+          const float cw_out =
+              P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(ce[k]));
+          const float cw = huber_sel(P, ce[k], w) ? w : cw_out;
+          const float ac = ca[k] * cw;
+          const double p1 = (double)(-ce[k] * ac);
+          const float p2 = ca[k] * ac;
+          t1 = k == 0 ? p1 : t1 + p1;
+          t2 = k == 0 ? p2 : t2 + p2;
+          nent += ca[k] > 0.0f ? 1u : 0u;
+        }
+        // the next voxel's entries and their (e, w), read before this
+        // voxel's e is written; independent of the sums, so they fill the
+        // reduction's latency
+        const int npix = s + 1 < G ? apix[s + 1] : bpix[0];
+        int nw0[VPL], nat[VPL];
+        float2 ewn[VPL];
+        float cn_prev[VPL], na[VPL];
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          const Rec &nr = s + 1 < G ? arec[s + 1][k] : brec[0][k];
+          nw0[k] = w0_of(nr, npix, k);
+          nat[k] = entry_at(nw0[k], k);
+          ewn[k] = s_ew[nat[k]];
+          const int d = nw0[k] + ub - cw0[k];  // the next voxel's bin in this voxel's window
+          const float pd = pick(crec[k], d), pu = pick(nr, ub);
+          cn_prev[k] = ((nw0[k] >= 0) & (cw0[k] >= 0)) ? pd : 0.0f;
+          na[k] = nw0[k] >= 0 ? pu : 0.0f;
+          // computed here, not sunk past the barrier onto the critical path
+          asm volatile("" ::"v"(cn_prev[k]), "v"(na[k]), "v"(nat[k]));
+        }
         wave_total_df(t1, t2);
         if (lane == 0) {
           S.t1[par][wave] = t1;
           S.t2[par][wave] = t2;
         }
-        nent += ca > 0.0f ? 1 : 0;
-        // next voxel's entry and its (e, w) while the other waves finish
-        const Rec &nr = s + 1 < G ? arec[s + 1] : brec[0];
-        const int nw0 = w0_of(nr, s + 1 < G ? apix[s + 1] : bpix[0]);
-        const int nat = entry_at(nw0);
-        const float2 ewn = s_ew[nat];
-        const int d = nw0 + ub - cw0;  // the next voxel's bin in this voxel's window
-        const float cn_prev = (nw0 >= 0 && cw0 >= 0) ? pick(crec, d) : 0.0f;
         __syncthreads();
         float V;
         bool skip;
         const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
         const float dx = xn - V;
-        s_ew[cat].x = fmaf(-ca, dx, e);
-        crec = nr;
-        cw0 = nw0;
-        ca = nw0 >= 0 ? pick(nr, ub) : 0.0f;
-        cat = nat;
-        ews = ewn;
-        cprev = cn_prev;
-        dxprev = dx;
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          s_ew[cat[k]].x = fmaf(-ca[k], dx, ce[k]);
+          ce[k] = fmaf(-cn_prev[k], dx, ewn[k].x);
+          cwv[k] = ewn[k].y;
+          crec[k] = s + 1 < G ? arec[s + 1][k] : brec[0][k];
+          cw0[k] = nw0[k];
+          ca[k] = na[k];
+          cat[k] = nat[k];
+        }
       }
 #pragma unroll
-      for (int k = 0; k < G; k++) {
-        apix[k] = bpix[k];
-        arec[k] = brec[k];
+      for (int j = 0; j < G; j++) {
+        apix[j] = bpix[j];
+#pragma unroll
+        for (int k = 0; k < VPL; k++) arec[j][k] = brec[j][k];
       }
 #pragma unroll
-      for (int k = 0; k < G; k++)
-        pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+      for (int j = 0; j < G; j++)
+        pnext[j] = (g + 1) * G + j < cnt ? __builtin_amdgcn_readlane(raw, j) : kNoPix;
       if (g == 0) {
-        // the first voxel's entry
-        crec = arec[0];
-        cw0 = w0_of(crec, apix[0]);
-        ca = cw0 >= 0 ? pick(crec, ub) : 0.0f;
-        cat = entry_at(cw0);
-        ews = s_ew[cat];
+        // the first voxel's entries
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          crec[k] = arec[0][k];
+          cw0[k] = w0_of(crec[k], apix[0], k);
+          ca[k] = cw0[k] >= 0 ? pick(crec[k], ub) : 0.0f;
+          cat[k] = entry_at(cw0[k], k);
+          const float2 ew0 = s_ew[cat[k]];
+          ce[k] = ew0.x;
+          cwv[k] = ew0.y;
+        }
       }
     }
     const double ne = wave_sum_d((double)nent);
@@ -851,11 +900,15 @@ __global__ __launch_bounds__(1024) void k_icd_mw(KP P, const int2 *__restrict__ 
   }
 }
 
-template __global__ void k_icd_mw<3, 1, 4>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mw<3, 1, 8>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mw<3, 1, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mw<5, 2, 8>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mw<5, 2, 15>(KP, const int2 *, int, int, int, int, int);
+// views per wave: VW * VPL = 21 * VPL (3 bins x 1 row) or 6 * VPL (5 bins x 2 rows)
+template __global__ void k_icd_mw<3, 1, 1, 4>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<3, 1, 2, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<3, 1, 2, 3>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<5, 2, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<5, 2, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw<5, 2, 2, 3>(KP, const int2 *, int, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Multi-wavefront voxel update with the error sinogram left in global memory
