@@ -138,9 +138,14 @@ def time_to_converged(scan, finest_cap: int = 400, tol: float = 1e-4):
                 else:
                     break
         st.synchronize()
+        ks = st.stats()
         stages.append(dict(mult=[mxy, mz], rule_stop_iteration=rule_it,
                            iterations_run=it, seconds=time.perf_counter() - ts,
-                           var_est=st.var_est))
+                           var_est=st.var_est, voxel_kernel_ms=ks["voxel_ms"],
+                           other_kernels_ms=ks["other_ms"], updates=ks["updates"],
+                           voxel_launches=ks["voxel_launches"],
+                           lds_launches=ks["lds_launches"],
+                           global_launches=ks["global_launches"]))
         if not finest:
             prev = dict(obj=st.object(), nz=st.info["N_z"], mag=st.magmap(), off=st.offset())
             var_est = st.var_est
@@ -182,6 +187,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--converge", action="store_true",
                     help="also measure the time to the converged MAP cost (plain init)")
+    ap.add_argument("--finest-cap", type=int, default=400,
+                    help="finest-stage iterations run by --converge past the stop rule")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -256,7 +263,7 @@ def main():
     }
     if args.converge:
         log("time to converged ...")
-        line["time_to_converged"] = time_to_converged(scan)
+        line["time_to_converged"] = time_to_converged(scan, finest_cap=args.finest_cap)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)

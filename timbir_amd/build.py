@@ -35,8 +35,8 @@ def build(verbose: bool = False) -> str:
         if verbose and text.strip():
             print(text)
     tmp = LIB + ".tmp"
-    subprocess.run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp],
-                   check=True)
+    subprocess.run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs,
+                    "-o", tmp], check=True)
     os.replace(tmp, LIB)
     return LIB
 

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "timbir_dev.h"
@@ -163,10 +164,11 @@ static int grid_for(size_t n, int block = 256) {
 // host restatements of the stage set-up
 // ---------------------------------------------------------------------------
 // XT_Profile.c:47-176: trapezoid voxel profile convolved with the normalised
-// Hamming beam, sampled every OffsetR, one row of 65 values per view.
+// Hamming beam, sampled every OffsetR, one row of 65 values per view.  Views
+// are independent; they are split over host threads (each row is computed
+// by the same sequential code whichever thread runs it).
 static void detector_response(const float *angles, int N_p, double delta_xy, double OffsetR,
                               double beam_width, float *H_r) {
-  std::vector<float> prof(kProfileRes);
   double beam[kBeamRes], bsum = 0;
   for (int i = 0; i < kBeamRes; i++) {
     beam[i] = 0.54 - 0.46 * cos((2 * kPi / kBeamRes) * i);
@@ -175,35 +177,45 @@ static void detector_response(const float *angles, int N_p, double delta_xy, dou
   for (int i = 0; i < kBeamRes; i++) beam[i] /= bsum;
   const double r0 = -beam_width / 2, step = beam_width / kBeamRes;
   const double tconst = kProfileRes / (2 * delta_xy);
-  for (int k = 0; k < N_p; k++) {
-    double ang = angles[k];
-    while (ang > kPi2) ang -= kPi2;
-    while (ang < 0) ang += kPi2;
-    const double peak = (ang <= kPi4) ? delta_xy / cos(ang) : delta_xy / cos(kPi2 - ang);
-    const double c45 = cos(kPi4);
-    const double d1 = c45 * cos(kPi4 - ang), d2 = c45 * fabs(cos(kPi4 + ang));
-    const double lc = 1 - d1, ln = 1 - d2, rn = 1 + d2, rc = 1 + d1;
-    for (int j = 0; j < kProfileRes; j++) {
-      const double t = 2.0 * j / kProfileRes;
-      double v;
-      if (t <= lc || t >= rc) v = 0;
-      else if (t > rn) v = peak * (rc - t) / (rc - rn);
-      else if (t >= ln) v = peak;
-      else v = peak * (t - lc) / (ln - lc);
-      prof[j] = (float)v;
-    }
-    for (int i = 0; i < kRespBins; i++) {
-      const double rmin = i * OffsetR - delta_xy;
-      double s = 0;
-      for (int q = 0; q < kBeamRes; q++) {
-        const double r = r0 + q * step;
-        const int idx = (int)floor((r - rmin) * tconst);
-        if (idx >= 0 && idx <= kProfileRes - 1) s += prof[idx] * beam[q];
+  auto rows = [&](int k0, int k1) {
+    std::vector<float> prof(kProfileRes);
+    for (int k = k0; k < k1; k++) {
+      double ang = angles[k];
+      while (ang > kPi2) ang -= kPi2;
+      while (ang < 0) ang += kPi2;
+      const double peak = (ang <= kPi4) ? delta_xy / cos(ang) : delta_xy / cos(kPi2 - ang);
+      const double c45 = cos(kPi4);
+      const double d1 = c45 * cos(kPi4 - ang), d2 = c45 * fabs(cos(kPi4 + ang));
+      const double lc = 1 - d1, ln = 1 - d2, rn = 1 + d2, rc = 1 + d1;
+      for (int j = 0; j < kProfileRes; j++) {
+        const double t = 2.0 * j / kProfileRes;
+        double v;
+        if (t <= lc || t >= rc) v = 0;
+        else if (t > rn) v = peak * (rc - t) / (rc - rn);
+        else if (t >= ln) v = peak;
+        else v = peak * (t - lc) / (ln - lc);
+        prof[j] = (float)v;
       }
-      H_r[(size_t)k * kHrStride + i] = (float)s;
+      for (int i = 0; i < kRespBins; i++) {
+        const double rmin = i * OffsetR - delta_xy;
+        double s = 0;
+        for (int q = 0; q < kBeamRes; q++) {
+          const double r = r0 + q * step;
+          const int idx = (int)floor((r - rmin) * tconst);
+          if (idx >= 0 && idx <= kProfileRes - 1) s += prof[idx] * beam[q];
+        }
+        H_r[(size_t)k * kHrStride + i] = (float)s;
+      }
+      H_r[(size_t)k * kHrStride + kRespBins] = 0.0f;
     }
-    H_r[(size_t)k * kHrStride + kRespBins] = 0.0f;
-  }
+  };
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int nt = std::max(1, std::min({hw > 0 ? hw : 1, 16, (N_p + 31) / 32}));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++)
+    pool.emplace_back(rows, (int)((long long)N_p * t / nt), (int)((long long)N_p * (t + 1) / nt));
+  rows(0, (int)((long long)N_p / nt));
+  for (auto &th : pool) th.join();
 }
 
 extern "C" void tb_stage_destroy(tb_stage *st) {
