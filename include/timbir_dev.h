@@ -51,6 +51,10 @@ typedef struct {
                                      fits, 4 = the multi-wavefront kernel
                                      with e in global memory where it
                                      applies */
+  int32_t views_per_lane;         /* views per theta-wave lane of the
+                                     multi-wavefront LDS kernel: 0 = the
+                                     fewest waves that keep one wavefront
+                                     per SIMD, else 1 or 2 */
 } tb_options;
 
 void tb_get_options(tb_options *opt);

@@ -314,7 +314,9 @@ def test_phases_track_oracle(kernel, c=1.0):
     (1, 1, 200, 2),   # 100 views per frame: two per lane
     (2, 2, 40, 2),    # second-finest stage: 5 bins x 2 rows per view
     (2, 2, 200, 2),
+    (2, 2, 128, 2),   # 64 views per frame: two views per lane, three theta waves
     (4, 4, 40, 2),    # 9 bins x 4 rows per view (e in global memory only)
+    (4, 4, 128, 2),   # 64 views per frame: four views per lane, three theta waves
     (4, 4, 200, 2),   # 100 views per frame: 15 theta waves
     (4, 4, 300, 2),   # 150 views per frame: two views per lane
 ])
@@ -323,7 +325,7 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
     if mxy == 4 and kernel in (2, 3):
         pytest.skip("the LDS kernels take columns of at most 5 bins x 2 rows")
     log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
-    n, nz = (64, 8) if mxy == 4 else (32, 4)
+    n, nz = (64, 8) if mxy == 4 or n_views == 128 else (32, 4)
     scan = scan_small(N=n, nz=nz, n_views=n_views, n_frames=n_frames, c_s=1.0, c_t=1.0)
     stage = 1 if mxy > 1 else 0
     o, g = pair(scan, stage_idx=stage, mult_xy=mxy, mult_z=mz,

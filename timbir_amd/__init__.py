@@ -32,7 +32,7 @@ class TbOptions(C.Structure):
     _fields_ = [("compute_cost", C.c_int32), ("abort_on_cost_increase", C.c_int32),
                 ("max_iterations", C.c_int32), ("no_nhicd", C.c_int32),
                 ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32),
-                ("kernel", C.c_int32)]
+                ("kernel", C.c_int32), ("views_per_lane", C.c_int32)]
 
 
 class TbStats(C.Structure):

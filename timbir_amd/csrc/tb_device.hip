@@ -54,7 +54,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -868,18 +868,20 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     const int vw = 64 / (nb * st->mult_z);
     const int nv = st->max_nv;
     int vpl = 1, nth = (nv + vw - 1) / vw;
-    if (nth > 3 && (nv + 2 * vw - 1) / (2 * vw) <= 3) {
+    const bool two = g_opt.views_per_lane == 2 ||
+                     (g_opt.views_per_lane == 0 && nth > 3 && (nv + 2 * vw - 1) / (2 * vw) <= 3);
+    if (two) {
       vpl = 2;
       nth = (nv + 2 * vw - 1) / (2 * vw);
     }
     const int vpw = (nv + nth - 1) / nth;
     mw_fn fn = nullptr;
     if (nb == 3) {
-      if (vpl == 2) fn = nth <= 2 ? k_icd_mw<3, 1, 2, 2> : k_icd_mw<3, 1, 2, 3>;
+      if (vpl == 2) fn = nth <= 2 ? k_icd_mw<3, 1, 2, 2> : nth <= 3 ? k_icd_mw<3, 1, 2, 3> : nullptr;
       else fn = nth <= 4 ? k_icd_mw<3, 1, 1, 4> : nth <= 8 ? k_icd_mw<3, 1, 1, 8>
               : nth <= 15 ? k_icd_mw<3, 1, 1, 15> : nullptr;
     } else {
-      if (vpl == 2) fn = k_icd_mw<5, 2, 2, 3>;
+      if (vpl == 2) fn = nth <= 3 ? k_icd_mw<5, 2, 2, 3> : nullptr;
       else fn = nth <= 8 ? k_icd_mw<5, 2, 1, 8> : nth <= 15 ? k_icd_mw<5, 2, 1, 15> : nullptr;
     }
     const size_t lds = icd_mw_lds_bytes(nv, st->N_r, st->mult_z, nth * vpl * 64);
@@ -887,18 +889,19 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
   }
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 4)) {
     // lane = (view, bin) with the slice's mult_z rows: NB bins, VW = 64 / NB
-    // views per lane layer, VPL layers
+    // views per lane layer, VPL layers.  The fewest views per lane that fit
+    // in 15 theta waves: these waves are latency-bound, and more of them
+    // sharing a SIMD hide each other's waits (measured on the C2 coarse
+    // stages: two or four views per lane in fewer waves ran 10-40% slower)
     const int w = 2 * st->mult_xy + 1;
     const int gnb = w <= 3 ? 3 : w <= 5 ? 5 : w <= 9 ? 9 : 0;
     const int mz = st->mult_z;
-    int vpl = 0;
     mw_fn fn = nullptr;
     int nth = 0, vpw = 0;
-    for (int tryv = 1; tryv <= 2 && !fn && gnb && st->N_r >= gnb; tryv++) {
-      const int cap = (64 / gnb) * tryv;
+    for (int vpl = 1; vpl <= 2 && !fn && gnb && st->N_r >= gnb; vpl++) {
+      const int cap = (64 / gnb) * vpl;
       nth = (st->max_nv + cap - 1) / cap;
       vpw = (st->max_nv + nth - 1) / nth;
-      vpl = tryv;
       if (nth > 15) continue;
       if (gnb == 3 && mz == 1 && vpl == 1)
         fn = nth <= 8 ? k_icd_mwg<3, 1, 1, 8> : k_icd_mwg<3, 1, 1, 15>;
@@ -909,6 +912,8 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     }
     if (fn && run_mw(fn, gnb, nth, vpw, 0, true) < 0) return -1;
   }
+
+
   if (!launched && nb && g_opt.kernel != 1) {
     using lds_fn = void (*)(KP, const int2 *, int, int, int, int);
     lds_fn fast = nullptr;

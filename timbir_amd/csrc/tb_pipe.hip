@@ -803,12 +803,28 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mw(KP P, const int2 *_
         // this voxel's partial sums (XT_VoxUpdate.c:61-92)
         double t1;
         float t2;
+        // generalized-Huber weights (XT_VoxUpdate.c:66-80): w inside the
+        // threshold, T delta sqrt(w) / |e| outside; the outer weight is
+        // evaluated only when some lane of the wave needs it
+        float cwk[VPL];
+        bool sel[VPL], allsel = true;
 #pragma unroll
         for (int k = 0; k < VPL; k++) {
-          const float w = cwv[k];
-          const float cw_out =
-              P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(ce[k]));
-          const float cw = huber_sel(P, ce[k], w) ? w : cw_out;
+          sel[k] = huber_sel(P, ce[k], cwv[k]);
+          cwk[k] = cwv[k];
+          allsel &= sel[k];
+        }
+        if (__builtin_expect(__ballot(!allsel) != 0ull, 0)) {
+#pragma unroll
+          for (int k = 0; k < VPL; k++) {
+            const float cw_out =
+                P.Td * __builtin_amdgcn_sqrtf(cwv[k]) * __builtin_amdgcn_rcpf(fabsf(ce[k]));
+            cwk[k] = sel[k] ? cwv[k] : cw_out;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          const float cw = cwk[k];
           const float ac = ca[k] * cw;
           const double p1 = (double)(-ce[k] * ac);
           const float p2 = ca[k] * ac;
@@ -962,7 +978,7 @@ __device__ __forceinline__ void st_rows(float *p, const float (&o)[MZ]) {
 }
 
 template <int NB, int MZ, int VPL, int MAXTH>
-__global__ __launch_bounds__(1024) void k_icd_mwg(KP P, const int2 *__restrict__ cells, int iter,
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *__restrict__ cells, int iter,
                                                   int chunk, int seq, int count_mag, int vpw) {
   constexpr int G = 8;             // pixels read per list load
   constexpr int VW = 64 / NB;      // views of one lane layer
@@ -1078,19 +1094,41 @@ __global__ __launch_bounds__(1024) void k_icd_mwg(KP P, const int2 *__restrict__
           ld_rows<MZ>(P.w + nea[k], nwv[k]);
         }
         // voxel n's theta sums (XT_VoxUpdate.c:61-92)
-        double t1 = 0.0;
-        float t2 = 0.0f;
+        // generalized-Huber weights (XT_VoxUpdate.c:66-80); the outer
+        // weight only when some lane of the wave needs it
+        float cwk[VPL][MZ];
+        bool sel[VPL][MZ], allsel = true;
+#pragma unroll
+        for (int k = 0; k < VPL; k++)
+#pragma unroll
+          for (int m = 0; m < MZ; m++) {
+            sel[k][m] = huber_sel(P, ce[k][m], cwv[k][m]);
+            cwk[k][m] = cwv[k][m];
+            allsel &= sel[k][m];
+          }
+        if (__builtin_expect(__ballot(!allsel) != 0ull, 0)) {
+#pragma unroll
+          for (int k = 0; k < VPL; k++)
+#pragma unroll
+            for (int m = 0; m < MZ; m++) {
+              const float cw_out = P.Td * __builtin_amdgcn_sqrtf(cwv[k][m]) *
+                                   __builtin_amdgcn_rcpf(fabsf(ce[k][m]));
+              cwk[k][m] = sel[k][m] ? cwv[k][m] : cw_out;
+            }
+        }
+        double t1;
+        float t2;
 #pragma unroll
         for (int k = 0; k < VPL; k++) {
 #pragma unroll
           for (int m = 0; m < MZ; m++) {
-            const float e = ce[k][m], w = cwv[k][m];
-            const float cw_out =
-                P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
-            const float cw = huber_sel(P, e, w) ? w : cw_out;
+            const float e = ce[k][m];
+            const float cw = cwk[k][m];
             const float ac = ca[k] * cw;
-            t1 += (double)(-e * ac);
-            t2 += ca[k] * ac;
+            const double p1 = (double)(-e * ac);
+            const float p2 = ca[k] * ac;
+            t1 = (k == 0 && m == 0) ? p1 : t1 + p1;
+            t2 = (k == 0 && m == 0) ? p2 : t2 + p2;
           }
           nent += ca[k] > 0.0f ? MZ : 0;
         }
