@@ -44,3 +44,27 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(tb, "LIB_PATH", "/nonexistent/libtimbir_hip.so")
     with pytest.raises(tb.TimbirError):
         tb.lib()
+
+
+DRIVER = os.path.join(ROOT, "timbir_amd", "tb_bin_data")
+
+
+def test_bin_data_driver_rejects_missing_input(tmp_path):
+    """The bin_data driver reads its inputs before any GPU call: a missing
+    projections.bin ends it with status 1 and the reference's message."""
+    import subprocess
+    assert os.path.exists(DRIVER), "build() makes timbir_amd/tb_bin_data"
+    r = subprocess.run([DRIVER, "--proj_rows", "4", "--proj_cols", "16", "--proj_num", "8",
+                        "--recon_num", "1", "--vox_wid", "1", "--rot_center", "8"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "ERROR: read_BinFile" in r.stdout
+    assert "huber_delta = 0.500000, huber_T = 4.000000" in r.stdout  # bin_data/XT_Main.h
+
+
+def test_bin_data_driver_rejects_bad_arguments(tmp_path):
+    import subprocess
+    r = subprocess.run([DRIVER, "--proj_rows", "4", "extra"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1
+    assert "ERROR: read_command_line_args" in r.stdout

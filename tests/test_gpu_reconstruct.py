@@ -1,0 +1,187 @@
+"""The drop-in entry point reconstruct() (src/include/mbir4d.h:34) and the
+bin_data command-line driver on the GPU: the whole multi-resolution run
+against the oracle's, the files it writes, restart from status.bin, and the
+driver against the library it wraps."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import timbir_amd as tb
+from oracle import pyorc
+from timbir_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "timbir_amd", "tb_bin_data")
+# the library's defaults (tb_get_options before any change)
+DEFAULTS = dict(compute_cost=0, abort_on_cost_increase=0, max_iterations=1000, no_nhicd=0,
+                write_files=1, order_seed=0, verbose=1, kernel=0, views_per_lane=0)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def scan4d(**kw):
+    # c = 1 keeps the prior smooth, so the GPU and oracle trajectories agree
+    # to FP32 over whole stages (see test_gpu_parity.test_phases_track_oracle)
+    base = dict(N=32, nz=4, n_frames=2, n_views=40, sig_s=0.005, sig_t=0.01, c_s=1.0, c_t=1.0,
+                convg_thresh=0.0, rings=0.01, remove_rings=1, quad_convex=0)
+    base.update(kw)
+    return datagen.tiny(**base)
+
+
+@pytest.fixture
+def options():
+    o = tb.get_options()
+    saved = {k: getattr(o, k) for k, _ in o._fields_}
+    tb.set_options(**DEFAULTS)
+    yield
+    tb.set_options(**saved)
+
+
+def n_stages(scan):
+    # XT_Engine.c:111-141 for these sizes: 32 columns give the (2,2), (1,1) schedule
+    return 2
+
+
+def test_reconstruct_follows_oracle(tmp_path, monkeypatch, options):
+    """Two-stage reconstruct() with ring correction, the zinger model and the
+    variance update: the final object against the oracle's whole
+    reconstruct() on the same schedule, 4 iterations per stage, NHICD off.
+    (NHICD picks lines by an order statistic of the update magnitudes; FP32
+    rounding of those magnitudes can move a near-tie across the threshold,
+    which changes the visiting order and the iterates by ~1e-3 while both
+    runs head for the same MAP point: test_reconstruct_reaches_oracle_map.)
+    The files reconstruct() writes hold the object it returns."""
+    monkeypatch.chdir(tmp_path)
+    log("reconstruct vs oracle")
+    scan = scan4d()
+    tb.set_options(max_iterations=4, verbose=0, no_nhicd=1)
+    x = tb.reconstruct(scan)
+    xo, _, _ = pyorc.reconstruct(scan, pyorc.options(schedule=pyorc.SCHED_COLOUR,
+                                                     max_iterations=4, no_nhicd=1))
+    rmse = float(np.sqrt(np.mean((x - xo) ** 2)) / (xo.max() - xo.min()))
+    log(f"  rmse/range {rmse:.2e}")
+    assert x.shape == (scan["recon_num"], scan["proj_rows"], scan["proj_cols"],
+                       scan["proj_cols"])
+    assert rmse < 1e-5
+    for f in range(scan["recon_num"]):
+        got = np.fromfile(f"object_time_{f}.bin", np.float32).reshape(x.shape[1:])
+        np.testing.assert_array_equal(got, x[f])
+    assert np.fromfile("status.bin", np.int32)[0] == n_stages(scan) - 1
+    assert os.path.getsize("variance_estimate.bin") == 8  # a double (Appendix quirk 13)
+    assert os.path.getsize("proj_offset.bin") == 4 * scan["proj_rows"] * scan["proj_cols"]
+    assert os.path.getsize("mag_update.bin") == 4 * x.size
+
+
+def test_reconstruct_equals_stage_pipeline(tmp_path, monkeypatch, options):
+    """With NHICD on, reconstruct() equals, bit for bit, the stage API driven
+    through the same schedule: stage runs, the multires object / map handoff,
+    offsets and variance carried over (XT_Engine.c:156-182).  Each stage's
+    NHICD path is held to the oracle by the stage-level tests."""
+    monkeypatch.chdir(tmp_path)
+    log("reconstruct vs the stage pipeline")
+    scan = scan4d()
+    tb.set_options(max_iterations=5, verbose=0)
+    x = tb.reconstruct(scan)
+    g0 = tb.Stage(scan, mult_xy=2, mult_z=2, stage_idx=0)
+    g0.stage_run(5, with_cost=False)
+    g1 = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=2, var_est=g0.var_est)
+    g1.init_from_coarse(g0.object(), g0.info["N_z"], g0.magmap())
+    g1.set_offset(g0.offset())
+    g0.close()
+    g1.stage_run(5, with_cost=False)
+    np.testing.assert_array_equal(x, g1.object())
+    assert float(np.fromfile("variance_estimate.bin", np.float64)[0]) == g1.var_est
+    np.testing.assert_array_equal(np.fromfile("proj_offset.bin", np.float32),
+                                  g1.offset().reshape(-1))
+    g1.close()
+
+
+def test_restart_resumes_after_last_stage(tmp_path, monkeypatch, options):
+    """status.bin = 0 and the stage-0 files: restart runs only the finest stage
+    and ends where an uninterrupted run ends, bit for bit."""
+    log("restart")
+    scan = scan4d()
+    tb.set_options(max_iterations=3, verbose=0)
+    full = tmp_path / "full"
+    part = tmp_path / "part"
+    full.mkdir()
+    part.mkdir()
+    monkeypatch.chdir(full)
+    x_full = tb.reconstruct(scan)
+    # the files an interrupted run leaves after stage 0, from the stage API
+    monkeypatch.chdir(part)
+    st = tb.Stage(scan, mult_xy=2, mult_z=2, stage_idx=0)
+    st.stage_run(3)
+    obj, mag = st.object(), st.magmap()
+    for f in range(scan["recon_num"]):
+        obj[f].tofile(f"object_time_{f}.bin")
+    mag.tofile("mag_update.bin")
+    st.offset().tofile("proj_offset.bin")
+    np.array([st.var_est], np.float64).tofile("variance_estimate.bin")
+    np.array([0], np.int32).tofile("status.bin")
+    st.close()
+    x_restart = tb.reconstruct(scan, restart=1)
+    np.testing.assert_array_equal(x_restart, x_full)
+    assert np.fromfile("status.bin", np.int32)[0] == 1
+    # every stage done: restart returns the last stage's object
+    np.testing.assert_array_equal(tb.reconstruct(scan, restart=1), x_full)
+
+
+def write_bin_inputs(scan, d):
+    f32 = np.float32
+    np.ascontiguousarray(scan["projections"], f32).tofile(d / "projections.bin")
+    np.ascontiguousarray(scan["weights"], f32).tofile(d / "weights.bin")
+    np.ascontiguousarray(scan["proj_angles"], f32).tofile(d / "proj_angles.bin")
+    np.ascontiguousarray(scan["proj_times"], f32).tofile(d / "proj_times.bin")
+    np.ascontiguousarray(scan["recon_times"], f32).tofile(d / "recon_times.bin")
+
+
+def driver_args(scan):
+    s = scan
+    a = ["--proj_rows", s["proj_rows"], "--proj_cols", s["proj_cols"], "--proj_num",
+         s["proj_num"], "--recon_num", s["recon_num"], "--vox_wid", s["vox_wid"],
+         "--rot_center", s["rot_center"], "--sig_s", s["sig_s"], "--sig_t", s["sig_t"],
+         "--c_s", s["c_s"], "--c_t", s["c_t"], "--convg_thresh", s["convg_thresh"],
+         "--remove_rings", s["remove_rings"], "--huber_delta", s["huber_delta"],
+         "--huber_T", s["huber_T"]]
+    if s["quad_convex"]:
+        a.append("--quad_convex")
+    return [DRIVER] + [repr(float(v)) if isinstance(v, float) else str(v) for v in a]
+
+
+def test_bin_data_driver_matches_library(tmp_path, monkeypatch, options):
+    """The driver reads the bin_data files and calls reconstruct() with the
+    library's default options: its object_time_*.bin equal those of a
+    reconstruct() call from Python on the same arrays."""
+    log("bin_data driver")
+    # float32-exact parameters, so the command line carries them unchanged
+    scan = scan4d(convg_thresh=2.0, sig_s=0.005, sig_t=0.01)
+    for k in ("vox_wid", "rot_center", "sig_s", "sig_t", "c_s", "c_t", "convg_thresh",
+              "huber_delta", "huber_T"):
+        scan[k] = float(np.float32(scan[k]))
+    d1 = tmp_path / "driver"
+    d2 = tmp_path / "library"
+    d1.mkdir()
+    d2.mkdir()
+    write_bin_inputs(scan, d1)
+    r = subprocess.run(driver_args(scan), cwd=d1, capture_output=True, text=True, timeout=240)
+    log(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    monkeypatch.chdir(d2)
+    tb.set_options(verbose=0)
+    x = tb.reconstruct(scan)
+    for f in range(scan["recon_num"]):
+        a = np.fromfile(d1 / f"object_time_{f}.bin", np.float32)
+        b = np.fromfile(d2 / f"object_time_{f}.bin", np.float32)
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(b.reshape(x.shape[1:]), x[f])
+    for name in ("proj_offset.bin", "variance_estimate.bin", "status.bin"):
+        assert (d1 / name).read_bytes() == (d2 / name).read_bytes(), name

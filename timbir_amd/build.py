@@ -38,6 +38,13 @@ def build(verbose: bool = False) -> str:
     subprocess.run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *objs,
                     "-o", tmp], check=True)
     os.replace(tmp, LIB)
+    # the bin_data command-line driver, linked against the library
+    exe = os.path.join(HERE, "tb_bin_data")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(CSRC, "tb_bin_main.cpp"), "-L" + HERE, "-ltimbir_hip",
+                    "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", exe + ".tmp"],
+                   check=True)
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 

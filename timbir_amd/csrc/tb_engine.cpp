@@ -147,7 +147,9 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
       return -1;
     }
     first = last + 1;
-    if (first < nres) {
+    {
+      // the files of the last completed stage: the next stage starts from
+      // them, or they are the result when every stage is done
       const int Nx = proj_cols / sxy[last], Nz = proj_rows / sz[last];
       const size_t plane = (size_t)Nz * Nx * Nx;
       prev_obj.resize(plane * recon_num);

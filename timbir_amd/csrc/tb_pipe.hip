@@ -427,19 +427,58 @@ __device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int pa
   return skip ? V : V + step;
 }
 
+// A-matrix column of a voxel in one view (calcAMatrixColumnforAngle,
+// XT_AMatrix.c:50-110) in bin units: u = (r - R0) / delta_r of the voxel
+// centre is xx * Au - yy * Bu + Cu; the column covers bins
+// floor(u) - m .. floor(u) + m (m = delta_xy / delta_r) clipped to the
+// detector, and its value at bin j is the lerp of H_r at |u - (j + 0.5)| in
+// units of OffsetR.  Contraction is off, so every caller (the column table,
+// the kernels that evaluate columns on the fly) gets the same bits.
+__device__ __forceinline__ void col_coef(const KP &P, int v, double &Au, double &Bu,
+                                         double &Cu) {
+#pragma clang fp contract(off)
+  const double cs = P.cosv[v], sn = P.sinv[v];
+  Au = P.delta_xy * cs * P.u_inv;
+  Bu = P.delta_xy * sn * P.u_inv;
+  Cu = ((P.x0 + 0.5 * P.delta_xy) * cs - (P.y0 + 0.5 * P.delta_xy) * sn - P.R0) * P.u_inv;
+}
+__device__ __forceinline__ double col_u(int xx, int yy, double Au, double Bu, double Cu) {
+  return fma((double)xx, Au, fma(-(double)yy, Bu, Cu));
+}
+// Window start w0 = min(first bin, N_r - nb) of a column of at most nb bins,
+// or -1 when the voxel misses the detector; loc / hic are the clipped first
+// and last bins.
+__device__ __forceinline__ int col_window(const KP &P, double u, int nb, int &loc, int &hic) {
+  const int m = P.mxy;
+  const bool on = (u + m >= 0.0) & (u - m <= P.u_max);
+  const int lo = (int)floor(u) - m;
+  loc = max(lo, 0);
+  hic = min(lo + 2 * m, P.N_r - 1);
+  return on ? min(loc, P.N_r - nb) : -1;
+}
+__device__ __forceinline__ float col_value(const KP &P, const float *H, double u, int j, int loc,
+                                           int hic) {
+#pragma clang fp contract(off)
+  const double t = fabs(u - ((double)j + 0.5)) * P.u_kk;
+  const int bin = (int)t;
+  const bool ok = (j >= loc) & (j <= hic) & (bin < kRespBins);
+  const int bb = ok ? bin : 0;
+  const float fr = (float)(t - (double)bin);
+  const float h0 = H[bb], h1 = H[bb + 1];
+  return fmaf(fr, h1 - h0, h0) * (ok ? 1.0f : 0.0f);
+}
+
 // Column table of the multi-wavefront kernel: for every (frame, pixel, view
-// slot) the A-matrix values at bins w0 .. w0 + NB - 1 of that view
-// (calcAMatrixColumnforAngle, XT_AMatrix.c:50-110, in the bin-unit form
-// above) and w0, or w0 = -1 when the voxel misses the detector in that view
-// or the slot holds no view.  Records of col_rec(NB) floats (NB + 1 rounded up
-// to a multiple of 4: 4, 8, 12 for NB = 3, 5, 9).
+// slot) the A-matrix values at bins w0 .. w0 + NB - 1 of that view and w0,
+// or w0 = -1 when the voxel misses the detector in that view or the slot
+// holds no view.  Records of col_rec(NB) floats (NB + 1 rounded up to a
+// multiple of 4: 4, 8, 12 for NB = 3, 5, 9).
 template <int NB>
 __global__ __launch_bounds__(256) void k_build_columns(KP P) {
   constexpr int REC = col_rec(NB);
   const int Nxy = P.N_x * P.N_y;
   const size_t nrec = (size_t)P.N_time * Nxy * P.colt_nv;
   float *out = const_cast<float *>(P.colt);
-  const int m = P.mxy;
   for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < nrec;
        idx += (size_t)gridDim.x * blockDim.x) {
     const int q = (int)(idx % P.colt_nv);
@@ -452,28 +491,16 @@ __global__ __launch_bounds__(256) void k_build_columns(KP P) {
     for (int ub = 0; ub < NB; ub++) a[ub] = 0.0f;
     if (q < nv) {
       const int v = P.view_idx[vb + q];
-      const double cs = P.cosv[v], sn = P.sinv[v];
-      const double Au = P.delta_xy * cs * P.u_inv, Bu = P.delta_xy * sn * P.u_inv;
-      const double Cu =
-          ((P.x0 + 0.5 * P.delta_xy) * cs - (P.y0 + 0.5 * P.delta_xy) * sn - P.R0) * P.u_inv;
+      double Au, Bu, Cu;
+      col_coef(P, v, Au, Bu, Cu);
       const int yy = lin / P.N_x, xx = lin - yy * P.N_x;
-      const double u = fma((double)xx, Au, fma(-(double)yy, Bu, Cu));
-      if (u + m >= 0.0 && u - m <= P.u_max) {
-        const int lo = (int)floor(u) - m;
-        const int loc = max(lo, 0), hic = min(lo + 2 * m, P.N_r - 1);
-        w0 = min(loc, P.N_r - NB);
+      const double u = col_u(xx, yy, Au, Bu, Cu);
+      int loc, hic;
+      w0 = col_window(P, u, NB, loc, hic);
+      if (w0 >= 0) {
         const float *H = P.hr + (size_t)v * kHrStride;
 #pragma unroll
-        for (int ub = 0; ub < NB; ub++) {
-          const int j = w0 + ub;
-          const double t = fabs(u - ((double)j + 0.5)) * P.u_kk;
-          const int bin = (int)t;
-          const bool ok = (j >= loc) & (j <= hic) & (bin < kRespBins);
-          const int bb = ok ? bin : 0;
-          const float fr = (float)(t - (double)bin);
-          const float h0 = H[bb], h1 = H[bb + 1];
-          a[ub] = fmaf(fr, h1 - h0, h0) * (ok ? 1.0f : 0.0f);
-        }
+        for (int ub = 0; ub < NB; ub++) a[ub] = col_value(P, H, u, w0 + ub, loc, hic);
       }
     }
     float *r = out + idx * REC;
