@@ -982,10 +982,10 @@ extern "C" int tb_update_colour(tb_stage *st, int32_t colour, int32_t iter, int3
 
 static int vsc_async(tb_stage *st) {
   KP P = st->kp();
-  hipLaunchKernelGGL(k_vsc_filter, dim3(grid_for(st->nobj())), dim3(256), 0, st->stream, P,
-                     st->filt);
+  hipLaunchKernelGGL(k_vsc_filter, dim3((unsigned)((st->nxy() + 255) / 256), st->N_time * st->N_z),
+                     dim3(256), 0, st->stream, P, st->filt);
   // NHICDSelectNum = floor(UpdateSelectNum / NHICD_Iterations), XT_NHICD.c:182
-  hipLaunchKernelGGL(k_vsc_select, dim3(st->N_time * st->N_z), dim3(256), 0, st->stream, P,
+  hipLaunchKernelGGL(k_vsc_select, dim3(st->N_time * st->N_z), dim3(1024), 0, st->stream, P,
                      st->filt, st->U / kNhicdK);
   HIPCHK(hipGetLastError());
   return 0;

@@ -514,120 +514,157 @@ __global__ void k_cost_prior(KP P, double sSq, double sSqp, double cs, double sT
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// ---------------------------------------------------------------------------
-// NHICD (XT_NHICD.c:128-216): 5x5 Hamming-weighted filter of each cell's
-// magnitude map, the k-th smallest value by 4-pass radix select, then the
-// pixels strictly above it in raster order.
-// ---------------------------------------------------------------------------
-__global__ void k_vsc_filter(KP P, float *__restrict__ filt) {
-  const double hw[5] = {0.0064, 0.0432, 0.0800, 0.0432, 0.0064};
-  const int Nxy = P.N_x * P.N_y;
-  const size_t n = (size_t)P.N_time * P.N_z * Nxy;
-  // the reference table is the outer product row x row / 0.08^2 ... but
-  // stored literally; keep its exact entries
+// 5x5 Hamming-weighted sum of the update-magnitude map (ComputeVSC,
+// XT_NHICD.c:128-162), one plane (frame, slice) per blockIdx.y.  Taps are
+// added in the reference's row-major order in double with contraction off,
+// so every filtered value, and with it the NHICD selection, equals the
+// CPU restatement's.
+__global__ __launch_bounds__(256) void k_vsc_filter(KP P, float *__restrict__ filt) {
+#pragma clang fp contract(off)
   const double tab[5][5] = {{0.0064, 0.0432, 0.0800, 0.0432, 0.0064},
                             {0.0432, 0.2916, 0.5400, 0.2916, 0.0432},
                             {0.0800, 0.5400, 1.0000, 0.5400, 0.0800},
                             {0.0432, 0.2916, 0.5400, 0.2916, 0.0432},
                             {0.0064, 0.0432, 0.0800, 0.0432, 0.0064}};
-  (void)hw;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int k = (int)(i % P.N_x);
-    const int j = (int)((i / P.N_x) % P.N_y);
-    const float *m = P.mag + (i / Nxy) * Nxy;
-    double acc = 0;
-    for (int a = -2; a <= 2; a++)
-      for (int c = -2; c <= 2; c++)
-        if (j + a >= 0 && j + a < P.N_y && k + c >= 0 && k + c < P.N_x)
-          acc += tab[a + 2][c + 2] * m[(size_t)(j + a) * P.N_x + (k + c)];
-    filt[i] = (float)acc;
-  }
+  const int Nxy = P.N_x * P.N_y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Nxy) return;
+  const int j = i / P.N_x, k = i - j * P.N_x;
+  const float *m = P.mag + (size_t)blockIdx.y * Nxy;
+  double acc = 0;
+#pragma unroll
+  for (int a = -2; a <= 2; a++)
+#pragma unroll
+    for (int c = -2; c <= 2; c++)
+      if (j + a >= 0 && j + a < P.N_y && k + c >= 0 && k + c < P.N_x)
+        acc += tab[a + 2][c + 2] * (double)m[(j + a) * P.N_x + (k + c)];
+  filt[(size_t)blockIdx.y * Nxy + i] = (float)acc;
 }
 
-// one workgroup (256 threads) per cell
-__global__ __launch_bounds__(256) void k_vsc_select(KP P, const float *__restrict__ filt,
-                                                    int want) {
+// NHICD line selection of one cell per workgroup of 1024 threads
+// (VSC_based_Voxel_Line_Select, XT_NHICD.c:161-216): the threshold is the
+// (N_xy - want)-th smallest filtered magnitude, found exactly by a 4-pass
+// 8-bit radix select over the float bits (the values are >= 0, so their bit
+// patterns order like the values), and the lines above it are listed in
+// raster order.  The reference finds the same order statistic with a
+// randomized quickselect (XT_NHICD.c:63-128).
+constexpr int kVscThreads = 1024;
+__global__ __launch_bounds__(kVscThreads) void k_vsc_select(KP P, const float *__restrict__ filt,
+                                                            int want) {
+  constexpr int NW = kVscThreads / 64;
   const int cell = blockIdx.x;
   const int Nxy = P.N_x * P.N_y;
   const uint32_t *vals = reinterpret_cast<const uint32_t *>(filt + (size_t)cell * Nxy);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   __shared__ uint32_t hist[256];
   __shared__ uint32_t s_prefix, s_k;
-  // k-th smallest, 1-based (RandomizedSelect(A, 0, n-1, n - want))
+  __shared__ int s_cnt[2][NW];
   if (threadIdx.x == 0) {
     s_prefix = 0;
-    s_k = (uint32_t)(Nxy - want);
+    s_k = (uint32_t)(Nxy - want);  // 1-based rank of the threshold
   }
   __syncthreads();
-  uint32_t thresh_bits;
-  if (want <= 0) {
-    thresh_bits = 0xffffffffu;  // nothing above
-  } else {
+  uint32_t thresh_bits = 0xffffffffu;  // want <= 0: nothing above
+  if (want > 0) {
     for (int pass = 0; pass < 4; pass++) {
       const int shift = 24 - 8 * pass;
       const uint32_t hmask = pass == 0 ? 0u : (0xffffffffu << (32 - 8 * pass));
-      hist[threadIdx.x] = 0;
+      if (threadIdx.x < 256) hist[threadIdx.x] = 0;
       __syncthreads();
       const uint32_t pre = s_prefix;
-      for (int i = threadIdx.x; i < Nxy; i += blockDim.x) {
+      for (int i = threadIdx.x; i < Nxy; i += kVscThreads) {
         const uint32_t b = vals[i];
         if ((b & hmask) == pre) atomicAdd(&hist[(b >> shift) & 0xff], 1u);
       }
       __syncthreads();
-      if (threadIdx.x == 0) {
-        uint32_t k = s_k, acc = 0;
-        int bucket = 0;
-        for (; bucket < 256; bucket++) {
-          if (acc + hist[bucket] >= k) break;
-          acc += hist[bucket];
+      if (wid == 0) {
+        // inclusive scan of the 256 counts, 4 per lane, then the first
+        // bucket whose running count reaches k
+        const uint32_t c0 = hist[4 * lane], c1 = hist[4 * lane + 1], c2 = hist[4 * lane + 2],
+                       c3 = hist[4 * lane + 3];
+        const uint32_t own = c0 + c1 + c2 + c3;
+        uint32_t incl = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t v = __shfl_up(incl, o);
+          if (lane >= o) incl += v;
         }
-        s_k = k - acc;
-        s_prefix = pre | ((uint32_t)bucket << shift);
+        const uint32_t excl = incl - own;
+        const uint32_t k = s_k;
+        const bool here = (excl < k) & (incl >= k);
+        const unsigned long long bal = __ballot(here);
+        const int owner = bal ? __ffsll((long long)bal) - 1 : 63;
+        if (lane == owner) {
+          uint32_t acc = excl;
+          int q = 0;
+          const uint32_t cs[4] = {c0, c1, c2, c3};
+          for (; q < 3; q++) {
+            if (acc + cs[q] >= k) break;
+            acc += cs[q];
+          }
+          s_k = k - acc;
+          s_prefix = pre | ((uint32_t)(4 * lane + q) << shift);
+        }
       }
       __syncthreads();
     }
     thresh_bits = s_prefix;
   }
   const float thresh = __uint_as_float(thresh_bits);
-  // compaction in raster order: wave ballots + one cross-wave prefix
-  __shared__ int s_wcnt[4];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int base = 0, base2 = 0;
-  for (int start = 0; start < Nxy; start += blockDim.x) {
-    const int i = start + threadIdx.x;
-    const bool flag = (want > 0) && i < Nxy && __uint_as_float(vals[i]) > thresh;
-    const unsigned long long bal = __ballot(flag);
-    const int inwave = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) s_wcnt[wid] = __popcll(bal);
-    __syncthreads();
-    int off = base;
-    for (int w = 0; w < wid; w++) off += s_wcnt[w];
-    if (flag) {
-      const int pos = off + inwave;
-      if (pos < P.nh_cap) P.nh_list[(size_t)cell * P.nh_cap + pos] = i;
+  // compaction in raster order: thread t owns the contiguous range
+  // [t * per, (t + 1) * per); counts, one block-wide exclusive scan, writes
+  const int per = (Nxy + kVscThreads - 1) / kVscThreads;
+  const int i0 = min(threadIdx.x * per, Nxy), i1 = min(i0 + per, Nxy);
+  int nsel = 0, nin = 0;
+  if (want > 0)
+    for (int i = i0; i < i1; i++) {
+      const bool f = __uint_as_float(vals[i]) > thresh;
+      nsel += f ? 1 : 0;
+      nin += (f && P.mask[i] == 1) ? 1 : 0;
     }
-    for (int w = 0; w < 4; w++) base += s_wcnt[w];
-    __syncthreads();
-    // the same lines inside the mask, for the pipelined update; a selected
-    // line outside the mask only has its map entry cleared (XT_VoxUpdate.c:156)
-    const bool in = flag && P.mask[i] == 1;
-    if (flag && !in) P.mag[(size_t)cell * Nxy + i] = 0.0f;
-    const unsigned long long bal2 = __ballot(in);
-    const int inwave2 = __popcll(bal2 & ((1ull << lane) - 1ull));
-    if (lane == 0) s_wcnt[wid] = __popcll(bal2);
-    __syncthreads();
-    int off2 = base2;
-    for (int w = 0; w < wid; w++) off2 += s_wcnt[w];
-    if (in) {
-      const int pos = off2 + inwave2;
-      if (pos < P.nh_cap) P.nh_ord[(size_t)cell * P.nh_cap + pos] = ((i / P.N_x) << 16) | (i % P.N_x);
+  // exclusive scans of (nsel, nin) across the block
+  int is = nsel, ii = nin;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int vs = __shfl_up(is, o), vi = __shfl_up(ii, o);
+    if (lane >= o) {
+      is += vs;
+      ii += vi;
     }
-    for (int w = 0; w < 4; w++) base2 += s_wcnt[w];
-    __syncthreads();
   }
+  if (lane == 63) {
+    s_cnt[0][wid] = is;
+    s_cnt[1][wid] = ii;
+  }
+  __syncthreads();
+  int bs = 0, bi = 0, ts = 0, ti = 0;
+  for (int w = 0; w < NW; w++) {
+    if (w < wid) {
+      bs += s_cnt[0][w];
+      bi += s_cnt[1][w];
+    }
+    ts += s_cnt[0][w];
+    ti += s_cnt[1][w];
+  }
+  int ps = bs + is - nsel, pi = bi + ii - nin;
+  if (want > 0)
+    for (int i = i0; i < i1; i++) {
+      if (!(__uint_as_float(vals[i]) > thresh)) continue;
+      if (ps < P.nh_cap) P.nh_list[(size_t)cell * P.nh_cap + ps] = i;
+      ps++;
+      if (P.mask[i] == 1) {
+        if (pi < P.nh_cap)
+          P.nh_ord[(size_t)cell * P.nh_cap + pi] = ((i / P.N_x) << 16) | (i % P.N_x);
+        pi++;
+      } else {
+        // a selected line outside the mask only has its map entry cleared
+        // (XT_VoxUpdate.c:156)
+        P.mag[(size_t)cell * Nxy + i] = 0.0f;
+      }
+    }
   if (threadIdx.x == 0) {
-    P.nh_count[cell] = min(base, P.nh_cap);
-    P.nh_ord_count[cell] = min(base2, P.nh_cap);
+    P.nh_count[cell] = min(ts, P.nh_cap);
+    P.nh_ord_count[cell] = min(ti, P.nh_cap);
   }
 }
 
