@@ -982,7 +982,7 @@ extern "C" int tb_update_colour(tb_stage *st, int32_t colour, int32_t iter, int3
 
 static int vsc_async(tb_stage *st) {
   KP P = st->kp();
-  hipLaunchKernelGGL(k_vsc_filter, dim3((unsigned)((st->nxy() + 255) / 256), st->N_time * st->N_z),
+  hipLaunchKernelGGL(k_vsc_filter, dim3((st->N_x + 15) / 16, (st->N_y + 15) / 16, st->N_time * st->N_z),
                      dim3(256), 0, st->stream, P, st->filt);
   // NHICDSelectNum = floor(UpdateSelectNum / NHICD_Iterations), XT_NHICD.c:182
   hipLaunchKernelGGL(k_vsc_select, dim3(st->N_time * st->N_z), dim3(1024), 0, st->stream, P,

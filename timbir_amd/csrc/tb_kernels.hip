@@ -515,10 +515,11 @@ __global__ void k_cost_prior(KP P, double sSq, double sSqp, double cs, double sT
 }
 
 // 5x5 Hamming-weighted sum of the update-magnitude map (ComputeVSC,
-// XT_NHICD.c:128-162), one plane (frame, slice) per blockIdx.y.  Taps are
-// added in the reference's row-major order in double with contraction off,
-// so every filtered value, and with it the NHICD selection, equals the
-// CPU restatement's.
+// XT_NHICD.c:128-162): one 16 x 16 tile of one plane (frame, slice) per
+// workgroup, staged in LDS with its 2-pixel halo.  Taps are added in the
+// reference's row-major order in double with contraction off, so every
+// filtered value, and with it the NHICD selection, equals the CPU
+// restatement's.
 __global__ __launch_bounds__(256) void k_vsc_filter(KP P, float *__restrict__ filt) {
 #pragma clang fp contract(off)
   const double tab[5][5] = {{0.0064, 0.0432, 0.0800, 0.0432, 0.0064},
@@ -526,19 +527,28 @@ __global__ __launch_bounds__(256) void k_vsc_filter(KP P, float *__restrict__ fi
                             {0.0800, 0.5400, 1.0000, 0.5400, 0.0800},
                             {0.0432, 0.2916, 0.5400, 0.2916, 0.0432},
                             {0.0064, 0.0432, 0.0800, 0.0432, 0.0064}};
+  __shared__ float tile[20][21];
   const int Nxy = P.N_x * P.N_y;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= Nxy) return;
-  const int j = i / P.N_x, k = i - j * P.N_x;
-  const float *m = P.mag + (size_t)blockIdx.y * Nxy;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
+  const float *m = P.mag + (size_t)blockIdx.z * Nxy;
+  for (int t = threadIdx.x; t < 400; t += 256) {
+    const int r = t / 20, c = t - r * 20;
+    const int y = y0 + r - 2, x = x0 + c - 2;
+    const bool in = (y >= 0) & (y < P.N_y) & (x >= 0) & (x < P.N_x);
+    tile[r][c] = in ? m[y * P.N_x + x] : 0.0f;
+  }
+  __syncthreads();
+  const int j = y0 + ty, k = x0 + tx;
+  if (j >= P.N_y || k >= P.N_x) return;
   double acc = 0;
 #pragma unroll
   for (int a = -2; a <= 2; a++)
 #pragma unroll
     for (int c = -2; c <= 2; c++)
       if (j + a >= 0 && j + a < P.N_y && k + c >= 0 && k + c < P.N_x)
-        acc += tab[a + 2][c + 2] * (double)m[(j + a) * P.N_x + (k + c)];
-  filt[(size_t)blockIdx.y * Nxy + i] = (float)acc;
+        acc += tab[a + 2][c + 2] * (double)tile[ty + 2 + a][tx + 2 + c];
+  filt[(size_t)blockIdx.z * Nxy + j * P.N_x + k] = (float)acc;
 }
 
 // NHICD line selection of one cell per workgroup of 1024 threads
@@ -572,9 +582,18 @@ __global__ __launch_bounds__(kVscThreads) void k_vsc_select(KP P, const float *_
       if (threadIdx.x < 256) hist[threadIdx.x] = 0;
       __syncthreads();
       const uint32_t pre = s_prefix;
-      for (int i = threadIdx.x; i < Nxy; i += kVscThreads) {
-        const uint32_t b = vals[i];
-        if ((b & hmask) == pre) atomicAdd(&hist[(b >> shift) & 0xff], 1u);
+      // eight independent loads in flight per thread
+      for (int i0 = threadIdx.x; i0 < Nxy; i0 += 8 * kVscThreads) {
+        uint32_t b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int i = i0 + u * kVscThreads;
+          b[u] = i < Nxy ? vals[i] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if ((i0 + u * kVscThreads < Nxy) && (b[u] & hmask) == pre)
+            atomicAdd(&hist[(b[u] >> shift) & 0xff], 1u);
       }
       __syncthreads();
       if (wid == 0) {
@@ -611,56 +630,53 @@ __global__ __launch_bounds__(kVscThreads) void k_vsc_select(KP P, const float *_
     thresh_bits = s_prefix;
   }
   const float thresh = __uint_as_float(thresh_bits);
-  // compaction in raster order: thread t owns the contiguous range
-  // [t * per, (t + 1) * per); counts, one block-wide exclusive scan, writes
-  const int per = (Nxy + kVscThreads - 1) / kVscThreads;
-  const int i0 = min(threadIdx.x * per, Nxy), i1 = min(i0 + per, Nxy);
+  // compaction in raster order: wave w owns the contiguous range
+  // [w * per, (w + 1) * per) and walks it 64 pixels at a time (coalesced,
+  // ballot offsets); one block-wide prefix of the wave counts in between
+  const int per = (Nxy + NW - 1) / NW;
+  const int w0 = min(wid * per, Nxy), w1 = min(w0 + per, Nxy);
   int nsel = 0, nin = 0;
   if (want > 0)
-    for (int i = i0; i < i1; i++) {
-      const bool f = __uint_as_float(vals[i]) > thresh;
-      nsel += f ? 1 : 0;
-      nin += (f && P.mask[i] == 1) ? 1 : 0;
+    for (int i = w0 + lane; i - lane < w1; i += 64) {
+      const bool f = i < w1 && __uint_as_float(vals[i]) > thresh;
+      const bool in = f && P.mask[i] == 1;
+      nsel += __popcll(__ballot(f));
+      nin += __popcll(__ballot(in));
     }
-  // exclusive scans of (nsel, nin) across the block
-  int is = nsel, ii = nin;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int vs = __shfl_up(is, o), vi = __shfl_up(ii, o);
-    if (lane >= o) {
-      is += vs;
-      ii += vi;
-    }
-  }
-  if (lane == 63) {
-    s_cnt[0][wid] = is;
-    s_cnt[1][wid] = ii;
+  if (lane == 0) {
+    s_cnt[0][wid] = nsel;
+    s_cnt[1][wid] = nin;
   }
   __syncthreads();
-  int bs = 0, bi = 0, ts = 0, ti = 0;
+  int ps = 0, pi = 0, ts = 0, ti = 0;
   for (int w = 0; w < NW; w++) {
     if (w < wid) {
-      bs += s_cnt[0][w];
-      bi += s_cnt[1][w];
+      ps += s_cnt[0][w];
+      pi += s_cnt[1][w];
     }
     ts += s_cnt[0][w];
     ti += s_cnt[1][w];
   }
-  int ps = bs + is - nsel, pi = bi + ii - nin;
+  const unsigned long long below = (1ull << lane) - 1ull;
   if (want > 0)
-    for (int i = i0; i < i1; i++) {
-      if (!(__uint_as_float(vals[i]) > thresh)) continue;
-      if (ps < P.nh_cap) P.nh_list[(size_t)cell * P.nh_cap + ps] = i;
-      ps++;
-      if (P.mask[i] == 1) {
-        if (pi < P.nh_cap)
-          P.nh_ord[(size_t)cell * P.nh_cap + pi] = ((i / P.N_x) << 16) | (i % P.N_x);
-        pi++;
-      } else {
-        // a selected line outside the mask only has its map entry cleared
-        // (XT_VoxUpdate.c:156)
-        P.mag[(size_t)cell * Nxy + i] = 0.0f;
+    for (int i = w0 + lane; i - lane < w1; i += 64) {
+      const bool f = i < w1 && __uint_as_float(vals[i]) > thresh;
+      const bool in = f && P.mask[i] == 1;
+      const unsigned long long bf = __ballot(f), bi = __ballot(in);
+      if (f) {
+        const int pos = ps + __popcll(bf & below);
+        if (pos < P.nh_cap) P.nh_list[(size_t)cell * P.nh_cap + pos] = i;
       }
+      if (in) {
+        const int pos = pi + __popcll(bi & below);
+        if (pos < P.nh_cap)
+          P.nh_ord[(size_t)cell * P.nh_cap + pos] = ((i / P.N_x) << 16) | (i % P.N_x);
+      }
+      // a selected line outside the mask only has its map entry cleared
+      // (XT_VoxUpdate.c:156)
+      if (f && !in) P.mag[(size_t)cell * Nxy + i] = 0.0f;
+      ps += __popcll(bf);
+      pi += __popcll(bi);
     }
   if (threadIdx.x == 0) {
     P.nh_count[cell] = min(ts, P.nh_cap);
