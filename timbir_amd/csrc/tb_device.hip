@@ -401,7 +401,10 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
   HIPCHK(dalloc(&st->cell_zero, ncell));
   HIPCHK(dalloc(&st->cell_ent, ncell));
   const int npix = st->N_r * st->N_t;
-  st->off_vpc = std::max(1, std::min(st->N_p, 64));
+  // 32 views per partial: the partial sums (npix x N_p / 32 of them) and
+  // the finalize pass over them, which adds them in chunk order, are both
+  // a few rounds of eight loads in flight
+  st->off_vpc = std::max(1, std::min(st->N_p, 32));
   st->off_chunks = (st->N_p + st->off_vpc - 1) / st->off_vpc;
   HIPCHK(dalloc(&st->off_pnum, (size_t)npix * st->off_chunks));
   HIPCHK(dalloc(&st->off_pden, (size_t)npix * st->off_chunks));

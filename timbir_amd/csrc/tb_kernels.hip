@@ -326,17 +326,28 @@ __global__ void k_offset_partial(KP P, int type, int views_per_chunk, double *__
     const double dof = P.d[(size_t)r * P.N_t + t];
     double num = 0, den = 0;
     const int v0 = chunk * views_per_chunk, v1 = min(P.N_p, v0 + views_per_chunk);
-    for (int v = v0; v < v1; v++) {
-      const size_t s = sidx(P, z, v, r, m);
-      const double e = P.e[s], w = P.w[s];
-      const double tmp = P.Td_d * sqrt(w);
-      const bool b = fabs(e * sqrt(w)) < P.T_d;
-      if (type == 3) {
-        if (b) { den += w; num += (e + dof) * w; }
-        else { den += tmp / fabs(e); num += (e + dof) * tmp / fabs(e); }
-      } else {
-        if (b) { num += e * w; den += w; }
-        else { double sg = (e > 0) - (e < 0); num += tmp * sg; den += tmp / fabs(e); }
+    // eight views' (e, w) loaded at once, then added in view order
+    for (int vb = v0; vb < v1; vb += 8) {
+      float ev[8], wv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const size_t s = sidx(P, z, min(vb + u, v1 - 1), r, m);
+        ev[u] = P.e[s];
+        wv[u] = P.w[s];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (vb + u >= v1) break;
+        const double e = ev[u], w = wv[u];
+        const double tmp = P.Td_d * sqrt(w);
+        const bool b = fabs(e * sqrt(w)) < P.T_d;
+        if (type == 3) {
+          if (b) { den += w; num += (e + dof) * w; }
+          else { den += tmp / fabs(e); num += (e + dof) * tmp / fabs(e); }
+        } else {
+          if (b) { num += e * w; den += w; }
+          else { double sg = (e > 0) - (e < 0); num += tmp * sg; den += tmp / fabs(e); }
+        }
       }
     }
     pnum[(size_t)chunk * npix + px] = num;
@@ -350,9 +361,21 @@ __global__ void k_offset_finalize(KP P, int nchunks, const double *__restrict__ 
   const int npix = P.N_r * P.N_t;
   for (int px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
     double a = 0, b = 0;
-    for (int c = 0; c < nchunks; c++) {
-      a += pnum[(size_t)c * npix + px];
-      b += pden[(size_t)c * npix + px];
+    // eight partials loaded at once, then added in chunk order
+    for (int cb = 0; cb < nchunks; cb += 8) {
+      double pa[8], pb[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int c = min(cb + u, nchunks - 1);
+        pa[u] = pnum[(size_t)c * npix + px];
+        pb[u] = pden[(size_t)c * npix + px];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (cb + u >= nchunks) break;
+        a += pa[u];
+        b += pb[u];
+      }
     }
     num[px] = a;
     den[px] = b;
