@@ -98,15 +98,15 @@ def multires_schedule(proj_rows: int, proj_cols: int):
     return list(zip(mxy[::-1], mz[::-1]))
 
 
-def time_to_converged(scan, finest_cap: int = 400, tol: float = 1e-4):
+def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4):
     """Plain (zero) initialisation under reconstruct()'s multires schedule on
     the GPU, every stage stopped by the reference's own rule (convergence
     ratio below convg_thresh after iteration 1, XT_ICD_update.c:1066-1104 and
     :1188): `seconds` is the wall time from the start of the first stage to
     that stop on the finest stage, with the MAP cost evaluated every finest
-    iteration.  The finest stage then runs on to finest_cap iterations so
-    that the cost trajectory shows how far that stop is from the MAP point:
-    `cost_gap_at_stop` = (C_stop - C_cap) / |C_cap|."""
+    iteration.  With finest_cap > 0 the finest stage then runs on to
+    finest_cap iterations so that the cost trajectory shows how far that stop
+    is from the MAP point: `cost_gap_at_stop` = (C_stop - C_cap) / |C_cap|."""
     import timbir_amd as tb
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     t0 = time.perf_counter()
@@ -125,7 +125,8 @@ def time_to_converged(scan, finest_cap: int = 400, tol: float = 1e-4):
         st.init_error_sino()
         finest = s == len(sched) - 1
         rule_it = None
-        for it in range(1, (finest_cap if finest else 1000) + 1):
+        cap = max(finest_cap, 1000 if finest_cap <= 0 else finest_cap) if finest else 1000
+        for it in range(1, cap + 1):
             conv, _ = st.icd_iteration(it)
             if finest:
                 c = st.cost()[0]
@@ -135,6 +136,8 @@ def time_to_converged(scan, finest_cap: int = 400, tol: float = 1e-4):
                 rule_it = it
                 if finest:
                     stop_t, stop_it = trace[-1][0], it
+                    if finest_cap <= 0:
+                        break
                 else:
                     break
         st.synchronize()
@@ -150,17 +153,19 @@ def time_to_converged(scan, finest_cap: int = 400, tol: float = 1e-4):
             prev = dict(obj=st.object(), nz=st.info["N_z"], mag=st.magmap(), off=st.offset())
             var_est = st.var_est
         st.close()
-    ccap = trace[-1][1]
     marks = [i for i in (1, 2, 5, 10, 20, 50, 100, 200, 400, 800) if i <= len(trace)]
     out = dict(seconds=stop_t, finest_rule_stop_iteration=stop_it, stages=stages,
-               finest_iterations_run=len(trace), c_cap=ccap,
-               cost_gap_at_stop=((trace[stop_it - 1][1] - ccap) / abs(ccap)) if stop_it else None,
-               finest_cost_at={str(i): trace[i - 1][1] for i in marks},
+               finest_iterations_run=len(trace), cost_at_stop=trace[stop_it - 1][1] if stop_it
+               else None, finest_cost_at={str(i): trace[i - 1][1] for i in marks},
                schedule=[list(x) for x in sched], init="zero (plain)",
                rule="convergence ratio < convg_thresh (reference rule)")
-    k = next((i for i, (t, c) in enumerate(trace) if abs(c - ccap) <= tol * abs(ccap)), None)
-    out["seconds_to_within_tol_of_c_cap"] = trace[k][0] if k is not None else None
-    out["tol"] = tol
+    if finest_cap > 0:
+        ccap = trace[-1][1]
+        out["c_cap"] = ccap
+        out["cost_gap_at_stop"] = ((trace[stop_it - 1][1] - ccap) / abs(ccap)) if stop_it else None
+        k = next((i for i, (t, c) in enumerate(trace) if abs(c - ccap) <= tol * abs(ccap)), None)
+        out["seconds_to_within_tol_of_c_cap"] = trace[k][0] if k is not None else None
+        out["tol"] = tol
     return out
 
 
@@ -185,10 +190,13 @@ def main():
     ap.add_argument("--small", action="store_true", help="tiny workload for a quick check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the time-to-converged run (plain init, reference stop rule)")
     ap.add_argument("--converge", action="store_true",
-                    help="also measure the time to the converged MAP cost (plain init)")
+                    help="run the finest stage on past the stop rule to --finest-cap "
+                         "iterations, to show the remaining cost gap")
     ap.add_argument("--finest-cap", type=int, default=400,
-                    help="finest-stage iterations run by --converge past the stop rule")
+                    help="finest-stage iterations run by --converge")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -261,9 +269,10 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None},
         "cost_after": cost,
     }
-    if args.converge:
+    if not args.no_converge and not args.small:
         log("time to converged ...")
-        line["time_to_converged"] = time_to_converged(scan, finest_cap=args.finest_cap)
+        line["time_to_converged"] = time_to_converged(
+            scan, finest_cap=args.finest_cap if args.converge else 0)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)
