@@ -120,8 +120,10 @@ def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4):
         st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
                       prev_mult_z=sched[s - 1][1] if s else 0, var_est=var_est)
         if prev is not None:
-            st.init_from_coarse(prev["obj"], prev["nz"], prev["mag"])
-            st.set_offset(prev["off"])
+            # the handoff of reconstruct(): upsampled from the previous stage in HBM
+            st.init_from_stage(prev)
+            st.set_offset(prev.offset())
+            prev.close()
         st.init_error_sino()
         finest = s == len(sched) - 1
         rule_it = None
@@ -150,9 +152,10 @@ def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4):
                            lds_launches=ks["lds_launches"],
                            global_launches=ks["global_launches"]))
         if not finest:
-            prev = dict(obj=st.object(), nz=st.info["N_z"], mag=st.magmap(), off=st.offset())
+            prev = st
             var_est = st.var_est
-        st.close()
+        else:
+            st.close()
     marks = [i for i in (1, 2, 5, 10, 20, 50, 100, 200, 400, 800) if i <= len(trace)]
     out = dict(seconds=stop_t, finest_rule_stop_iteration=stop_it, stages=stages,
                finest_iterations_run=len(trace), cost_at_stop=trace[stop_it - 1][1] if stop_it

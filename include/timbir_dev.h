@@ -91,10 +91,15 @@ int tb_set_offset(tb_stage *st, const float *d);
 int tb_get_offset(tb_stage *st, float *d);
 int tb_set_magmap(tb_stage *st, const float *mag);
 int tb_get_magmap(tb_stage *st, float *mag);
-/* initObject initICD = 2 / 3 from the previous stage's object and map, on
- * the GPU: bilinear x2 in x-y (and z when mult_z halves). */
+/* initObject initICD = 2 / 3 (XT_ICD_update.c:323-651) from the previous
+ * stage's object and map held by the caller (host arrays, as the reference
+ * reads them back from object_time_*.bin / mag_update.bin): bilinear x2 in
+ * x-y (and z when mult_z halves). */
 int tb_init_from_coarse(tb_stage *st, const float *prev_obj, int32_t prev_nz,
                         const float *prev_mag);
+/* The same handoff from a previous stage still in HBM (initObject without the
+ * file round trip of XT_Engine.c:156-182): same values, bit for bit. */
+int tb_init_from_stage(tb_stage *st, tb_stage *prev);
 int tb_get_error_sino(tb_stage *st, float *e);  /* [N_p][N_r][N_t] */
 int tb_get_weights(tb_stage *st, float *w);     /* [N_p][N_r][N_t] */
 int tb_get_detector_response(tb_stage *st, float *H_r); /* [N_p][65] */

@@ -104,6 +104,28 @@ def test_init_from_coarse_matches_oracle(prev_mz):
     np.testing.assert_array_equal(g1.magmap(), o1.mag())
 
 
+@pytest.mark.parametrize("prev_mz", [1, 2])
+def test_init_from_stage_matches_host_path(prev_mz):
+    """The handoff from a previous stage held in HBM (tb_init_from_stage)
+    gives the object and map of the host path (tb_init_from_coarse), bit
+    for bit, in the x-y and x-y-z forms."""
+    log(f"init from stage prev_mz={prev_mz}")
+    scan = scan_small(N=32, nz=4, n_views=40)
+    g0 = tb.Stage(scan, mult_xy=2, mult_z=prev_mz, stage_idx=0)
+    x0 = random_object(g0.obj_shape, np.ones(g0.obj_shape[2:], np.float32), seed=11)
+    m0 = np.random.default_rng(5).random(g0.obj_shape).astype(np.float32)
+    g0.set_object(x0)
+    g0.set_magmap(m0)
+    a = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=prev_mz)
+    b = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=prev_mz)
+    a.init_from_stage(g0)
+    b.init_from_coarse(x0, g0.info["N_z"], m0)
+    np.testing.assert_array_equal(a.object(), b.object())
+    np.testing.assert_array_equal(a.magmap(), b.magmap())
+    for st in (g0, a, b):
+        st.close()
+
+
 def test_forward_projection_bitwise():
     log("forward projection")
     scan = scan_small(N=32, nz=4, n_views=40)

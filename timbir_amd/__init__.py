@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libtimbir_hip.so")
 EXPORTS = [
     "reconstruct", "tb_get_options", "tb_set_options", "tb_last_error", "tb_stage_create",
     "tb_stage_destroy", "tb_stage_info", "tb_set_object", "tb_get_object", "tb_set_offset",
-    "tb_get_offset", "tb_set_magmap", "tb_get_magmap", "tb_init_from_coarse",
+    "tb_get_offset", "tb_set_magmap", "tb_get_magmap", "tb_init_from_coarse", "tb_init_from_stage",
     "tb_get_error_sino", "tb_get_weights", "tb_get_detector_response", "tb_get_var_est",
     "tb_init_error_sino", "tb_forward_project", "tb_cost", "tb_offset_update",
     "tb_variance_update", "tb_voxel_update", "tb_update_colour", "tb_vsc_select",
@@ -76,6 +76,7 @@ def lib() -> C.CDLL:
         getattr(L, n).argtypes = [vp, _f32p]
         getattr(L, n).restype = C.c_int
     L.tb_init_from_coarse.argtypes = [vp, _f32p, i32, _f32p]
+    L.tb_init_from_stage.argtypes = [vp, vp]
     L.tb_get_var_est.restype = C.c_double
     L.tb_get_var_est.argtypes = [vp]
     L.tb_init_error_sino.argtypes = [vp]
@@ -212,6 +213,9 @@ class Stage:
     def init_from_coarse(self, prev_obj, prev_nz, prev_mag):
         _check(self.L.tb_init_from_coarse(self.h, np.ascontiguousarray(prev_obj, np.float32),
                                           prev_nz, np.ascontiguousarray(prev_mag, np.float32)))
+
+    def init_from_stage(self, prev: "Stage"):
+        _check(self.L.tb_init_from_stage(self.h, prev.h))
 
     def init_error_sino(self):
         _check(self.L.tb_init_error_sino(self.h))

@@ -19,6 +19,7 @@ namespace tb {
 __global__ void k_transpose_in(KP, const float *, const float *, double);
 __global__ void k_to_ref_layout(KP, const float *, float *);
 __global__ void k_forward(KP, const int *, float *, int);
+__global__ void k_upsample(const float *, float *, int, int, int, int, int);
 template <int NT>
 __global__ void k_update_voxels(KP, const int2 *, int, int, int, int, int);
 template <int VPL, int NB, int MZ, int G>
@@ -658,6 +659,28 @@ extern "C" int tb_init_from_coarse(tb_stage *st, const float *prev_obj, int32_t 
     upsample(prev_mag, prev_nz, st->N_y / 2, st->N_x / 2, st->N_time, st->N_z, three, true, o);
     if (tb_set_magmap(st, o.data())) return -1;
   }
+  return 0;
+}
+
+// The same handoff from the previous stage's device buffers: the object and
+// the update-magnitude map are upsampled in HBM, no host copy.
+extern "C" int tb_init_from_stage(tb_stage *st, tb_stage *prev) {
+  if (!prev) return fail("tb_init_from_stage: no previous stage");
+  if (st->initICD != 2 && st->initICD != 3) return fail("tb_init_from_stage: stage 0");
+  const bool three = st->initICD == 3;
+  if (prev->N_time != st->N_time || 2 * prev->N_x != st->N_x || 2 * prev->N_y != st->N_y ||
+      (three ? 2 * prev->N_z : prev->N_z) != st->N_z)
+    return fail("tb_init_from_stage: previous stage %dx%dx%d does not upsample to %dx%dx%d",
+                prev->N_x, prev->N_y, prev->N_z, st->N_x, st->N_y, st->N_z);
+  HIPCHK(hipStreamSynchronize(prev->stream));
+  hipLaunchKernelGGL(k_upsample, dim3(grid_for(st->nobj())), dim3(256), 0, st->stream, prev->obj,
+                     st->obj, st->N_time, prev->N_z, prev->N_y, prev->N_x, three ? 1 : 0);
+  if (st->initMagUpMap)
+    hipLaunchKernelGGL(k_upsample, dim3(grid_for(st->nobj())), dim3(256), 0, st->stream,
+                       prev->mag, st->mag, st->N_time, prev->N_z, prev->N_y, prev->N_x,
+                       three ? 1 : 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st->stream));
   return 0;
 }
 

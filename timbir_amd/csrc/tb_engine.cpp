@@ -182,6 +182,7 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
   bool freed_inputs = false;
   std::vector<float> final_obj;
   int final_nz = 0;
+  tb_stage *prev_st = nullptr;  // the previous stage, kept in HBM for the handoff
   for (int s = first; s < nres; s++) {
     log_info(dbg, verbose,
              "Running multi-resolution stage %d with x-y voxel scale = %d and z voxel scale = "
@@ -193,6 +194,7 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
                         sig_t, c_s, c_t, convg_thresh, remove_rings, quad_convex, huber_delta,
                         huber_T, sxy[s], sz[s], s, s ? sz[s - 1] : 0, var_est)) {
       log_err(dbg, "%s\n", tb_last_error());
+      tb_stage_destroy(prev_st);
       return -1;
     }
     int32_t ints[16];
@@ -225,7 +227,14 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
       freed_inputs = true;
     }
     int rc = 0;
-    if (s > first || (s == first && !prev_obj.empty())) {
+    if (prev_st) {
+      // object and map upsampled from the previous stage in HBM
+      rc = tb_init_from_stage(st, prev_st);
+      tb_stage_destroy(prev_st);
+      prev_st = nullptr;
+      if (!rc && !prev_off.empty()) rc = tb_set_offset(st, prev_off.data());
+    } else if (s > first || (s == first && !prev_obj.empty())) {
+      // restart: the previous stage's files
       rc = tb_init_from_coarse(st, prev_obj.data(), prev_nz, prev_mag.data());
       if (!rc && !prev_off.empty()) rc = tb_set_offset(st, prev_off.data());
     }
@@ -244,14 +253,19 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
       write_bin("cost.bin", costs.data(), sizeof(double));
       for (int i = 1; i <= iters; i++) append_bin("cost.bin", &costs[i], sizeof(double));
     }
-    std::vector<float> obj(plane * recon_num), mag(plane * recon_num),
+    // host copies only for the stage's files and for the final result
+    const bool last = s == nres - 1;
+    const bool host = opt.write_files || last;
+    std::vector<float> obj(host ? plane * recon_num : 0), mag(host ? plane * recon_num : 0),
         off((size_t)proj_cols * proj_rows);
-    rc = tb_get_object(st, obj.data()) || tb_get_magmap(st, mag.data()) ||
-         tb_get_offset(st, off.data());
+    rc = tb_get_offset(st, off.data());
+    if (!rc && host) rc = tb_get_object(st, obj.data()) || tb_get_magmap(st, mag.data());
     var_est = tb_get_var_est(st);
-    tb_stage_destroy(st);
+    if (last) tb_stage_destroy(st);
+    else prev_st = st;
     if (rc) {
       log_err(dbg, "%s\n", tb_last_error());
+      tb_stage_destroy(prev_st);
       return -1;
     }
     log_info(dbg, verbose, "Time elapsed is %f minutes.\n", difftime(time(NULL), start) / 60.0);

@@ -44,6 +44,75 @@ __global__ void k_to_ref_layout(KP P, const float *__restrict__ src, float *__re
 }
 
 // ---------------------------------------------------------------------------
+// initObject's bilinear x2 upsampling (XT_ICD_update.c:323-651) from a coarser
+// stage's array, on the device: in is [N_time][prev_nz][Ny][Nx], out is
+// [N_time][out_nz][2 Ny][2 Nx] with out_nz = prev_nz (x-y, initICD 2) or
+// 2 prev_nz (x-y-z, initICD 3).  Every output value is evaluated with the
+// double expressions of the host restatement (x pass, y pass, z pass, edge
+// rows first), contraction off; in the x-y form the x pass is rounded to
+// float, as the reference's Real_arr_t rows are.  Bit for bit the host path.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double up_x(const float *r, int Nx, int k2, bool fmid) {
+#pragma clang fp contract(off)
+  const int W = 2 * Nx;
+  double v;
+  if (k2 == W - 1) v = r[Nx - 1];
+  else if (k2 == W - 2) v = (r[Nx - 2] + 3.0 * r[Nx - 1]) / 4.0;
+  else if (k2 == 0) v = r[0];
+  else if (k2 == 1) v = (3.0 * r[0] + r[1]) / 4.0;
+  else {
+    const int k = k2 >> 1;
+    v = (k2 & 1) ? (3.0 * r[k] + r[k + 1]) / 4.0 : (r[k - 1] + 3.0 * r[k]) / 4.0;
+  }
+  return fmid ? (double)(float)v : v;
+}
+
+__device__ __forceinline__ double up_xy(const float *plane, int Ny, int Nx, int j2, int k2,
+                                        bool fmid) {
+#pragma clang fp contract(off)
+  auto T = [&](int j) { return up_x(plane + (size_t)j * Nx, Nx, k2, fmid); };
+  const int H = 2 * Ny;
+  if (j2 == 0) return T(0);
+  if (j2 == 1) return (3.0 * T(0) + T(1)) / 4.0;
+  if (j2 == H - 1) return T(Ny - 1);
+  if (j2 == H - 2) return (T(Ny - 2) + 3.0 * T(Ny - 1)) / 4.0;
+  const int j = j2 >> 1;
+  return (j2 & 1) ? (3.0 * T(j) + T(j + 1)) / 4.0 : (T(j - 1) + 3.0 * T(j)) / 4.0;
+}
+
+__global__ void k_upsample(const float *__restrict__ in, float *__restrict__ out, int N_time,
+                           int prev_nz, int Ny, int Nx, int three) {
+#pragma clang fp contract(off)
+  const int W = 2 * Nx, H = 2 * Ny, out_nz = three ? 2 * prev_nz : prev_nz;
+  const size_t plane_in = (size_t)Ny * Nx, plane_out = (size_t)H * W;
+  const size_t n = (size_t)N_time * out_nz * plane_out;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int x2 = (int)(i % W);
+    const int y2 = (int)((i / W) % H);
+    const int z2 = (int)((i / plane_out) % out_nz);
+    const int f = (int)(i / (plane_out * out_nz));
+    const float *base = in + (size_t)f * prev_nz * plane_in;
+    double v;
+    if (!three) {
+      v = up_xy(base + (size_t)z2 * plane_in, Ny, Nx, y2, x2, true);
+    } else {
+      auto U = [&](int s) { return up_xy(base + (size_t)s * plane_in, Ny, Nx, y2, x2, false); };
+      const int Z = 2 * prev_nz;
+      if (z2 == 0) v = U(0);
+      else if (z2 == 1) v = (3.0 * U(0) + U(1)) / 4.0;
+      else if (z2 == Z - 1) v = U(prev_nz - 1);
+      else if (z2 == Z - 2) v = (3.0 * U(prev_nz - 1) + U(prev_nz - 2)) / 4.0;
+      else {
+        const int s2 = z2 >> 1;
+        v = (z2 & 1) ? (3.0 * U(s2) + U(s2 + 1)) / 4.0 : (U(s2 - 1) + 3.0 * U(s2)) / 4.0;
+      }
+    }
+    out[i] = (float)v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Forward projection, gather form: thread per (slice z, view v, bin r) sums
 // the voxels of the frame that owns v whose column covers r, in raster
 // order, accumulating exactly as forward_project_voxel_AMat1D
