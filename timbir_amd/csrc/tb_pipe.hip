@@ -376,9 +376,12 @@ __global__ __launch_bounds__(64) void k_icd_lds(KP P, const int2 *__restrict__ c
 // Multi-wavefront form: one workgroup per cell, split by role.
 //
 // Waves 0..nth-1 ("theta waves") each own vpw views of the cell, one A-matrix
-// entry (view, bin, row) per lane, and the LDS rows of those views: no other
-// wave reads or writes them, so the in-order LDS argument above still gives
-// the sequential sweep.  The last wave ("prior wave") owns the object side:
+// entry (view, bin, row) per lane for each of VPL views, and the LDS rows of
+// those views: no other wave reads or writes them, so the in-order LDS
+// argument above still gives the sequential sweep.  Where the views allow,
+// the workgroup has at most four waves, one per SIMD (two views per lane
+// for 64 views of 3 bins: 2 theta waves).  The last wave ("prior wave")
+// owns the object side:
 // neighbour prefetch one group ahead (with patching, as above), the qGGMRF
 // prior, and the object / map stores.  Per voxel every wave publishes its
 // partial sums in LDS and meets the others at one barrier; then every wave
