@@ -1,7 +1,8 @@
 """The drop-in entry point reconstruct() (src/include/mbir4d.h:34) and the
 bin_data command-line driver on the GPU: the whole multi-resolution run
-against the oracle's, the files it writes, restart from status.bin, and the
-driver against the library it wraps."""
+against the oracle's on each workload's data model, the files it writes,
+restart from status.bin, and the driver against the library it wraps."""
+import functools
 import os
 import subprocess
 import sys
@@ -11,12 +12,10 @@ import pytest
 
 import timbir_amd as tb
 from oracle import pyorc
-from timbir_amd import datagen
+from timbir_amd import bindata, datagen
 
 pytestmark = pytest.mark.gpu
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DRIVER = os.path.join(ROOT, "timbir_amd", "tb_bin_data")
 # the library's defaults (tb_get_options before any change)
 DEFAULTS = dict(compute_cost=0, abort_on_cost_increase=0, max_iterations=1000, no_nhicd=0,
                 write_files=1, order_seed=0, verbose=1, kernel=0, views_per_lane=0)
@@ -135,28 +134,6 @@ def test_restart_resumes_after_last_stage(tmp_path, monkeypatch, options):
     np.testing.assert_array_equal(tb.reconstruct(scan, restart=1), x_full)
 
 
-def write_bin_inputs(scan, d):
-    f32 = np.float32
-    np.ascontiguousarray(scan["projections"], f32).tofile(d / "projections.bin")
-    np.ascontiguousarray(scan["weights"], f32).tofile(d / "weights.bin")
-    np.ascontiguousarray(scan["proj_angles"], f32).tofile(d / "proj_angles.bin")
-    np.ascontiguousarray(scan["proj_times"], f32).tofile(d / "proj_times.bin")
-    np.ascontiguousarray(scan["recon_times"], f32).tofile(d / "recon_times.bin")
-
-
-def driver_args(scan):
-    s = scan
-    a = ["--proj_rows", s["proj_rows"], "--proj_cols", s["proj_cols"], "--proj_num",
-         s["proj_num"], "--recon_num", s["recon_num"], "--vox_wid", s["vox_wid"],
-         "--rot_center", s["rot_center"], "--sig_s", s["sig_s"], "--sig_t", s["sig_t"],
-         "--c_s", s["c_s"], "--c_t", s["c_t"], "--convg_thresh", s["convg_thresh"],
-         "--remove_rings", s["remove_rings"], "--huber_delta", s["huber_delta"],
-         "--huber_T", s["huber_T"]]
-    if s["quad_convex"]:
-        a.append("--quad_convex")
-    return [DRIVER] + [repr(float(v)) if isinstance(v, float) else str(v) for v in a]
-
-
 def test_bin_data_driver_matches_library(tmp_path, monkeypatch, options):
     """The driver reads the bin_data files and calls reconstruct() with the
     library's default options: its object_time_*.bin equal those of a
@@ -171,8 +148,9 @@ def test_bin_data_driver_matches_library(tmp_path, monkeypatch, options):
     d2 = tmp_path / "library"
     d1.mkdir()
     d2.mkdir()
-    write_bin_inputs(scan, d1)
-    r = subprocess.run(driver_args(scan), cwd=d1, capture_output=True, text=True, timeout=240)
+    bindata.write(scan, d1)
+    r = subprocess.run(bindata.command_line(scan), cwd=d1, capture_output=True, text=True,
+                       timeout=240)
     log(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
     monkeypatch.chdir(d2)
@@ -185,3 +163,71 @@ def test_bin_data_driver_matches_library(tmp_path, monkeypatch, options):
         np.testing.assert_array_equal(b.reshape(x.shape[1:]), x[f])
     for name in ("proj_offset.bin", "variance_estimate.bin", "status.bin"):
         assert (d1 / name).read_bytes() == (d2 / name).read_bytes(), name
+
+
+def follows_oracle(scan, iters=4):
+    """reconstruct() against the oracle's reconstruct() on the same schedule,
+    iters iterations per stage, NHICD off.  Returns rmse / range of the final
+    object against the FP32 oracle, against the FP64 oracle, and between the
+    two oracles (how far FP32 rounding alone moves this trajectory)."""
+    tb.set_options(max_iterations=iters, verbose=0, no_nhicd=1)
+    x = tb.reconstruct(scan)
+    opt = dict(schedule=pyorc.SCHED_COLOUR, max_iterations=iters, no_nhicd=1)
+    x32, _, _ = pyorc.reconstruct(scan, pyorc.options(**opt))
+    x64, _, _ = pyorc.reconstruct(scan, pyorc.options(**opt), precision="f64")
+    assert x.shape == x32.shape
+
+    def rel(a, b):
+        return float(np.sqrt(np.mean((a - b) ** 2)) / (b.max() - b.min()))
+    out = rel(x, x32), rel(x, x64), rel(x32, x64)
+    log("  rmse/range vs FP32 oracle %.2e, vs FP64 oracle %.2e, FP32 vs FP64 oracle %.2e" % out)
+    return out
+
+
+def check_workload(scan, iters=4):
+    """Two runs of one workload's data.  With c_s = c_t = 1 the prior is
+    smooth and FP32 rounding moves the trajectory by ~1e-7: the object must
+    equal the FP32 oracle's to 1e-5 of its range.  With the workload's own
+    prior (c = 1e-2 makes the qGGMRF curvature near zero differences large,
+    so rounding differences grow over the sweeps; the FP32 and FP64 oracles
+    part by ~1e-3 on these sizes) the object must be as close to the FP64
+    oracle as the FP32 oracle is, within a factor 3 (plus 1e-5 of the
+    range, the bar of the smooth run)."""
+    smooth = dict(scan, c_s=1.0, c_t=1.0)
+    r32, _, _ = follows_oracle(smooth, iters)
+    assert r32 < 1e-5
+    _, r64, ref = follows_oracle(scan, iters)
+    assert r64 <= 3 * ref + 1e-5
+
+
+def test_plain3d_follows_oracle(tmp_path, monkeypatch, options):
+    """The plain 3D workload at its full size (128 x 128 x 4, 256 views,
+    quadratic data term): two stages of 3 iterations."""
+    monkeypatch.chdir(tmp_path)
+    log("plain3d 128x128x4, 256 views")
+    check_workload(datagen.plain3d(convg_thresh=0.0), iters=3)
+
+
+@functools.lru_cache(maxsize=1)
+def batch():
+    # the batched workload's model at reduced sizes: uneven widths and views
+    return datagen.batched(n_problems=3, sizes=(32, 48, 64), views=(24, 36, 48),
+                           convg_thresh=0.0)
+
+
+@pytest.mark.parametrize("which", range(3))
+def test_batched_problem_follows_oracle(tmp_path, monkeypatch, options, which):
+    """One problem of the batched workload (one frame, 4 rows, its own width,
+    view count and I0) through reconstruct()."""
+    monkeypatch.chdir(tmp_path)
+    scan = batch()[which]
+    log(f"batched problem {which}: N={scan['proj_cols']} views={scan['proj_num']}")
+    check_workload(scan)
+
+
+def test_precision_scan_follows_oracle(tmp_path, monkeypatch, options):
+    """The precision-study model (microstructure plus ellipsoid sections,
+    ring offsets and ring correction) at 32 x 32 x 8, 48 views."""
+    monkeypatch.chdir(tmp_path)
+    log("precision3d 32x32x8, 48 views")
+    check_workload(datagen.precision3d(N=32, nz=8, n_views=48, convg_thresh=0.0))

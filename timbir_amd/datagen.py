@@ -132,6 +132,104 @@ def disk(N: int, frac=0.98) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# ellipses in detector units, with closed-form projections
+# ---------------------------------------------------------------------------
+#
+# An ellipse is a row (x0, y0, a, b, t, v) in the projector's coordinates
+# (pixel units times vox_wid, origin on the rotation axis): centre (x0, y0),
+# semi-axis a along (cos t, sin t), b along (-sin t, cos t), value v added
+# inside.
+
+
+def draw_ellipses(rng: np.random.Generator, R: float, n_min=3, n_max=12) -> np.ndarray:
+    """Seeded ellipses inside the disk of radius R: n_min..n_max of them,
+    semi-axes U(0.05, 0.5) R, centres U(-0.35, 0.35) R per coordinate (so the
+    whole ellipse stays inside R), angle U(0, pi), value U(0.1, 1)."""
+    n = int(rng.integers(n_min, n_max + 1))
+    out = np.empty((n, 6))
+    for q in range(n):
+        a, b = rng.uniform(0.05, 0.5, 2) * R
+        x0, y0 = rng.uniform(-0.35, 0.35, 2) * R
+        out[q] = (x0, y0, a, b, rng.uniform(0, np.pi), rng.uniform(0.1, 1.0))
+    return out
+
+
+def rasterize_ellipses(ell: np.ndarray, N: int, rot_center: float, vox_wid: float = 1.0,
+                       sub: int = 4) -> np.ndarray:
+    """[N][N] image: each pixel the mean over a sub x sub grid of points of
+    the summed ellipse values (pixel (i, k) centred at
+    x = (k + 0.5 - rot_center) vox_wid, y = (i + 0.5 - N/2) vox_wid)."""
+    off = (np.arange(sub) + 0.5) / sub - 0.5
+    xs = ((np.arange(N)[:, None] + 0.5 + off[None, :] - rot_center) * vox_wid).reshape(-1)
+    ys = ((np.arange(N)[:, None] + 0.5 + off[None, :] - N / 2) * vox_wid).reshape(-1)
+    X, Y = np.meshgrid(xs, ys)
+    img = np.zeros_like(X)
+    for x0, y0, a, b, t, v in ell:
+        u = (X - x0) * np.cos(t) + (Y - y0) * np.sin(t)
+        w = -(X - x0) * np.sin(t) + (Y - y0) * np.cos(t)
+        img += v * ((u / a) ** 2 + (w / b) ** 2 <= 1)
+    return img.reshape(N, sub, N, sub).mean(axis=(1, 3))
+
+
+def _chord_integrals(d, s2):
+    """First and second antiderivatives of the unit chord profile
+    sqrt(s^2 - d^2) (zero outside |d| < s), both zero at d = -s:
+    G(d) = (d sqrt(s^2 - d^2) + s^2 asin(d / s)) / 2 + pi s^2 / 4 and
+    H(d) = integral of G = ((2 s^2 + d^2) sqrt(s^2 - d^2)) / 6
+           + s^2 (d asin(d / s) + pi d / 2) / 2 + pi s^4 / 12 ... continued
+    linearly past d = s."""
+    sw = np.sqrt(s2)
+    dc = np.clip(d, -sw, sw)
+    rt = np.sqrt(np.maximum(s2 - dc * dc, 0))
+    asn = np.arcsin(dc / sw)
+    G = 0.5 * (dc * rt + s2 * asn) + 0.25 * np.pi * s2
+    # integral of G from -s to dc, then the constant G(s) = pi s^2 / 2 beyond s
+    H = (0.5 * (-(rt ** 3) / 3 + s2 * (dc * asn + rt)) + 0.25 * np.pi * s2 * dc
+         + 0.25 * np.pi * s2 * sw)
+    H = H + 0.5 * np.pi * s2 * np.maximum(d - sw, 0)
+    return G, H
+
+
+def project_ellipses(ell: np.ndarray, angles: np.ndarray, n_bins: int, rot_center: float,
+                     vox_wid: float = 1.0, profile: str = "triangle") -> np.ndarray:
+    """[n_views][n_bins] exact line integrals of the ellipses, weighted per
+    detector bin j (centre r_j = (j + 0.5 - rot_center) vox_wid, ray offset
+    r = x cos(theta) - y sin(theta)) by a profile of unit area:
+    "triangle" (half-width one bin: the linear interpolation that project()
+    applies to each sample, so both agree up to rasterization) or "box" (the
+    mean over the bin).
+
+    Along direction n the ellipse has half-width s with s^2 = a^2 (n.e1)^2 +
+    b^2 (n.e2)^2 and chord length (2ab / s^2) sqrt(s^2 - d^2) at offset d
+    from its centre's ray; the profile-weighted integrals of that are
+    differences of its antiderivatives G and H."""
+    th = np.asarray(angles, np.float64)[:, None]
+    c, s_ = np.cos(th), np.sin(th)
+    h = float(vox_wid)
+    centres = (np.arange(n_bins) + 0.5 - rot_center) * h
+    out = np.zeros((len(th), n_bins))
+    for x0, y0, a, b, t, v in ell:
+        # n = (cos th, -sin th); e1 = (cos t, sin t); e2 = (-sin t, cos t)
+        n1 = c * np.cos(t) - s_ * np.sin(t)
+        n2 = -c * np.sin(t) - s_ * np.cos(t)
+        s2 = (a * n1) ** 2 + (b * n2) ** 2
+        d = centres[None, :] - (x0 * c - y0 * s_)
+        if profile == "box":
+            G1, _ = _chord_integrals(d + 0.5 * h, s2)
+            G0, _ = _chord_integrals(d - 0.5 * h, s2)
+            w = (G1 - G0) / h
+        elif profile == "triangle":
+            _, Hp = _chord_integrals(d + h, s2)
+            _, Hc = _chord_integrals(d, s2)
+            _, Hm = _chord_integrals(d - h, s2)
+            w = (Hp - 2 * Hc + Hm) / (h * h)
+        else:
+            raise ValueError(f"unknown profile {profile!r}")
+        out += v * (2 * a * b / s2) * w
+    return out
+
+
+# ---------------------------------------------------------------------------
 # projection and noise
 # ---------------------------------------------------------------------------
 
@@ -274,3 +372,92 @@ def tiny(N=16, nz=4, n_frames=2, n_views=24, I0=2e4, seed=7, rings=0.0, zingers=
         zi = rng.choice(proj.size, max(1, int(zingers * proj.size)), replace=False)
         proj.reshape(-1)[zi] *= 5.0
     return _scan(proj.astype(np.float32), wts, angles, times, rt, nz, N, n_frames, truth, **kw)
+
+
+def _phantom_scale(raster_max: float, vmax: float) -> float:
+    return vmax / max(raster_max, 1e-12)
+
+
+def batched(n_problems=64, sizes=(128, 192, 256, 384, 512), views=(180, 360, 720), rows=4,
+            I0_range=(5e3, 5e4), vmax=0.02, seed=20260003, **kw) -> list:
+    """Batch of independent 3D problems of uneven size: per problem a square
+    image width N drawn from sizes (N bins), a view count from views (uniform
+    on [0, pi)), 3-12 seeded ellipses scaled to [0, vmax], and Poisson counts
+    at I0 ~ U(I0_range).  rows identical slices (the reference's slab
+    partition needs at least 4 rows), recon_num 1, quadratic data term.
+    Line integrals are the ellipses' closed-form projections with project()'s
+    triangular bin profile; ``truth`` is their 4x4-supersampled raster."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_problems):
+        N = int(rng.choice(sizes))
+        V = int(rng.choice(views))
+        I0 = float(rng.uniform(*I0_range))
+        ell = draw_ellipses(rng, 0.95 * N / 2)
+        img = rasterize_ellipses(ell, N, N / 2)
+        k = _phantom_scale(img.max(), vmax)
+        ell[:, 5] *= k
+        truth = np.repeat((img * k).astype(np.float32)[None], rows, axis=0)
+        angles = (np.arange(V) * (np.pi / V)).astype(np.float32)
+        li = project_ellipses(ell, angles, N, N / 2)
+        proj, wts = noisy_scan(rng, np.repeat(li[:, None, :], rows, axis=1), I0)
+        times = np.arange(V, dtype=np.float32)
+        rt = np.array([0, V], np.float32)
+        out.append(_scan(proj, wts, angles, times, rt, rows, N, 1, truth[None], I0=I0, **kw))
+    return out
+
+
+def precision3d(N=512, nz=32, n_views=1440, I0=2e4, ring_sigma=0.005, ch_steps=64,
+                seed=20260004, **kw) -> dict:
+    """Precision-study scan: one frame, N x N x nz, n_views uniform on
+    [0, pi), N bins, Poisson I0, per-detector ring offsets N(0, ring_sigma).
+    The phantom mixes a two-phase Cahn-Hilliard microstructure (0.002 /
+    0.008 inside the reconstruction disk, the same in every slice) with 3-12
+    seeded ellipsoids (axes in the slice plane from draw_ellipses, z centre
+    and half-height uniform over the slab) scaled so the ellipsoids reach
+    0.012: sections differ per slice.  The microstructure is projected by
+    project(), the ellipsoid sections in closed form with the same bin
+    profile.  Ring correction on, quadratic data term."""
+    rng = np.random.default_rng(seed)
+    ch = two_phase(cahn_hilliard(rng, N, ch_steps), 0.002, 0.008) * disk(N)
+    ell2 = draw_ellipses(rng, 0.95 * N / 2)
+    zc = rng.uniform(-nz / 2, nz / 2, len(ell2))
+    zh = rng.uniform(0.25, 1.0, len(ell2)) * nz
+    zs = np.arange(nz) + 0.5 - nz / 2
+    sections = []
+    for z in zs:
+        k2 = 1 - ((z - zc) / zh) ** 2
+        e = ell2[k2 > 0].copy()
+        e[:, 2:4] *= np.sqrt(k2[k2 > 0])[:, None]
+        sections.append(e)
+    rasters = np.stack([rasterize_ellipses(e, N, N / 2) for e in sections])
+    k = _phantom_scale(rasters.max(), 0.012)
+    truth = (ch[None] + rasters * k).astype(np.float32)
+    angles = (np.arange(n_views) * (np.pi / n_views)).astype(np.float32)
+    li = np.repeat(project(ch[None], angles, N, N / 2), nz, axis=1)
+    for s, e in enumerate(sections):
+        if len(e):
+            e = e.copy()
+            e[:, 5] *= k
+            li[:, s, :] += project_ellipses(e, angles, N, N / 2)
+    proj, wts = noisy_scan(rng, li, I0)
+    rings = rng.normal(0, ring_sigma, (nz, N)).astype(np.float32)
+    proj = proj + rings[None]
+    times = np.arange(n_views, dtype=np.float32)
+    rt = np.array([0, n_views], np.float32)
+    return _scan(proj, wts, angles, times, rt, nz, N, 1, truth[None], sig_s=0.01,
+                 remove_rings=1, quad_convex=1, rings=rings, **kw)
+
+
+def deploy4d(N=512, nz=32, n_frames=16, n_views=2048, K=8, I0=1e4, seed=20260005, **kw) -> dict:
+    """Deployment-size 4D interlaced scan: the interlaced4d model (evolving
+    Cahn-Hilliard microstructure, zingers, ring offsets, generalized-Huber
+    data term, ring correction) at N x N x nz, n_frames frames, n_views
+    interlaced views in K sub-frames, Poisson I0."""
+    return interlaced4d(N=N, nz=nz, n_frames=n_frames, n_views=n_views, K=K, I0=I0, seed=seed,
+                        **kw)
+
+
+# the workloads by name
+WORKLOADS = dict(plain3d=plain3d, interlaced4d=interlaced4d, batched=batched,
+                 precision3d=precision3d, deploy4d=deploy4d)
