@@ -126,9 +126,12 @@ def test_init_from_stage_matches_host_path(prev_mz):
         st.close()
 
 
-def test_forward_projection_bitwise():
-    log("forward projection")
-    scan = scan_small(N=32, nz=4, n_views=40)
+@pytest.mark.parametrize("N,nz", [(32, 4), (24, 12), (16, 16)])
+def test_forward_projection_bitwise(N, nz):
+    # 4, 12 and 16 slices: one partial group of 8, a full and a partial
+    # group, two full groups
+    log(f"forward projection N={N} nz={nz}")
+    scan = scan_small(N=N, nz=nz, n_views=40)
     o, g = pair(scan)
     x = random_object(o.obj_shape, o.mask())
     o.obj()[...] = x

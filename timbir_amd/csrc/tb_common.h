@@ -15,6 +15,8 @@ constexpr int kProfileRes = 1536;
 constexpr int kBeamRes = 512;
 constexpr int kRespBins = 64;  // DETECTOR_RESPONSE_BINS; rows hold 65 values
 constexpr int kHrStride = kRespBins + 1;
+constexpr int kFwdZ = 8;    // slices one forward-projection thread accumulates
+constexpr int kFwdT = 256;  // detector bins per forward-projection block
 constexpr double kAlpha = 1.5;
 constexpr int kMaxIters = 1000;
 constexpr double kZingerOffT = 100000.0;

@@ -18,7 +18,8 @@
 namespace tb {
 __global__ void k_transpose_in(KP, const float *, const float *, double);
 __global__ void k_to_ref_layout(KP, const float *, float *);
-__global__ void k_forward(KP, const int *, float *, int);
+__global__ void k_fwd_interleave(KP, float4 *);
+__global__ void k_forward(KP, const int *, const float4 *, float *, int);
 __global__ void k_upsample(const float *, float *, int, int, int, int, int);
 template <int NT>
 __global__ void k_update_voxels(KP, const int2 *, int, int, int, int, int);
@@ -709,24 +710,39 @@ static int harvest_events(tb_stage *st) {
 // ---------------------------------------------------------------------------
 // operations
 // ---------------------------------------------------------------------------
-extern "C" int tb_init_error_sino(tb_stage *st) {
+// Forward projection into `out` (mode 0: A x, mode 1: the error sinogram):
+// the object is copied slice-interleaved into a temporary buffer, then
+// k_forward runs a block per (view, tile of kFwdT bins) x group of kFwdZ
+// slices.  Returns after the stream has finished.
+static int run_forward(tb_stage *st, float *out, int mode) {
   KP P = st->kp();
-  const size_t n = (size_t)st->N_z * st->N_p * st->N_r;
-  hipLaunchKernelGGL(k_forward, dim3(grid_for(n, 128)), dim3(128), 0, st->stream, P,
-                     st->view_frame_d, st->e, 1);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(st->stream));
+  const int ngroups = (st->N_z + kFwdZ - 1) / kFwdZ;
+  const int ntile = (st->N_r + kFwdT - 1) / kFwdT;
+  const size_t nrec = (size_t)st->N_time * ngroups * st->N_x * st->N_y;
+  float4 *objt = nullptr;
+  HIPCHK(dalloc(&objt, 2 * nrec));
+  hipLaunchKernelGGL(k_fwd_interleave, dim3(grid_for(nrec)), dim3(256), 0, st->stream, P, objt);
+  hipLaunchKernelGGL(k_forward, dim3(ntile * st->N_p, ngroups), dim3(kFwdT), 0, st->stream, P,
+                     st->view_frame_d, objt, out, mode);
+  hipError_t err = hipGetLastError();
+  if (err == hipSuccess) err = hipStreamSynchronize(st->stream);
+  (void)hipFree(objt);
+  HIPCHK(err);
+  return 0;
+}
+
+extern "C" int tb_init_error_sino(tb_stage *st) {
+  if (run_forward(st, st->e, 1)) return -1;
   return 0;
 }
 
 extern "C" int tb_forward_project(tb_stage *st, float *out) {
   float *tmp = nullptr;
   HIPCHK(dalloc(&tmp, st->nsino()));
-  KP P = st->kp();
-  const size_t n = (size_t)st->N_z * st->N_p * st->N_r;
-  hipLaunchKernelGGL(k_forward, dim3(grid_for(n, 128)), dim3(128), 0, st->stream, P,
-                     st->view_frame_d, tmp, 0);
-  HIPCHK(hipGetLastError());
+  if (run_forward(st, tmp, 0)) {
+    (void)hipFree(tmp);
+    return -1;
+  }
   int rc = sino_to_host(st, tmp, out);
   (void)hipFree(tmp);
   return rc;

@@ -113,55 +113,119 @@ __global__ void k_upsample(const float *__restrict__ in, float *__restrict__ out
 }
 
 // ---------------------------------------------------------------------------
-// Forward projection, gather form: thread per (slice z, view v, bin r) sums
-// the voxels of the frame that owns v whose column covers r, in raster
-// order, accumulating exactly as forward_project_voxel_AMat1D
-// (XT_ForwardProject.c:8-25) does: float += double(pixel) * double(A).
-// Every detector row of the slice receives the same value.
-// mode 0: out = A x; mode 1: e = y - A x - d.
+// Forward projection, gather form.  Thread per (bin r, view v, group of
+// kFwdZ slices); block per kFwdT bins of one view.  Each entry sums the
+// voxels of the frame that owns v whose column covers r, in raster order,
+// accumulating exactly as forward_project_voxel_AMat1D
+// (XT_ForwardProject.c:8-25) does: float += double(pixel) * double(A), so
+// the result is bit-identical to the reference's.  The A entry depends only
+// on (view, pixel, bin): each is evaluated once and applied to the group's
+// slices, read as one 32-byte record from the slice-interleaved copy that
+// k_fwd_interleave writes.  Each thread visits only the rows and columns
+// whose centres project into reach of its bin, so the threads of a view do
+// similar work at every angle.  Every detector row of a slice receives the
+// same value.  mode 0: out = A x; mode 1: e = y - A x - d.  Blocks are
+// numbered view-major inside each XCD's share (block b runs on XCD b mod
+// 8), so one XCD's L2 holds the objects of few frames.
 // ---------------------------------------------------------------------------
-__global__ void k_forward(KP P, const int *__restrict__ view_frame, float *__restrict__ out,
-                          int mode) {
-  size_t n = (size_t)P.N_z * P.N_p * P.N_r;
+
+// objt[f][g][row][col][q] = obj[f][g * kFwdZ + q][row][col], zero past N_z
+__global__ void k_fwd_interleave(KP P, float4 *__restrict__ objt) {
+  const int ngroups = (P.N_z + kFwdZ - 1) / kFwdZ;
+  const size_t Nxy = (size_t)P.N_x * P.N_y;
+  const size_t n = (size_t)P.N_time * ngroups * Nxy;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    int r = (int)(i % P.N_r);
-    int v = (int)((i / P.N_r) % P.N_p);
-    int z = (int)(i / ((size_t)P.N_r * P.N_p));
-    int f = view_frame[v];
-    float acc = 0.0f;
-    if (f >= 0) {
-      const float *H = P.hr + (size_t)v * kHrStride;
-      const double c = P.cosv[v], s = P.sinv[v];
-      const float *obj = P.obj + ((size_t)f * P.N_z + z) * P.N_x * P.N_y;
-      // columns whose projected centre lies in [lo, hi) can cover bin r
-      double lo = P.R0 + r * P.delta_r - P.delta_xy - 1e-9;
-      double hi = P.R0 + (r + 1) * P.delta_r + P.delta_xy + 1e-9;
-      for (int row = 0; row < P.N_y; row++) {
-        double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
-        int k0 = 0, k1 = P.N_x - 1;
-        if (fabs(c) > 1e-6) {
-          // x*c - y*s in [lo, hi)  =>  x in [(lo + y s)/c, (hi + y s)/c]
-          double xa = (lo + y * s) / c, xb = (hi + y * s) / c;
-          if (xa > xb) { double t = xa; xa = xb; xb = t; }
-          double ka = (xa - P.x0) / P.delta_xy - 0.5, kb = (xb - P.x0) / P.delta_xy - 0.5;
-          // clamp before converting so huge ranges cannot overflow an int
-          ka = fmin(fmax(ka, -4.0), (double)P.N_x + 4.0);
-          kb = fmin(fmax(kb, -4.0), (double)P.N_x + 4.0);
-          k0 = max(0, (int)floor(ka) - 1);
-          k1 = min(P.N_x - 1, (int)ceil(kb) + 1);
-        }
-        for (int col = k0; col <= k1; col++) {
-          float px = obj[(size_t)row * P.N_x + col];
-          double a = acol_bin(P, H, c, s, row, col, r);
-          if (a != 0.0) acc = (float)((double)acc + (double)px * a);
+    const size_t pix = i % Nxy, fg = i / Nxy;
+    const int g = (int)(fg % ngroups), f = (int)(fg / ngroups);
+    float v[kFwdZ];
+#pragma unroll
+    for (int q = 0; q < kFwdZ; q++) {
+      const int z = g * kFwdZ + q;
+      v[q] = z < P.N_z ? P.obj[((size_t)f * P.N_z + z) * Nxy + pix] : 0.0f;
+    }
+    objt[2 * i] = make_float4(v[0], v[1], v[2], v[3]);
+    objt[2 * i + 1] = make_float4(v[4], v[5], v[6], v[7]);
+  }
+}
+
+// first and last index of the cells (centres at o + (k + 0.5) h, k in
+// [0, n)) whose centre t satisfies t * g in [lo, hi], padded by one cell
+__device__ __forceinline__ void fwd_span(double lo, double hi, double g, double o, double h,
+                                         int n, int &k0, int &k1) {
+  double ta = lo / g, tb = hi / g;
+  if (ta > tb) { double t = ta; ta = tb; tb = t; }
+  double ka = (ta - o) / h - 0.5, kb = (tb - o) / h - 0.5;
+  // clamp before converting so huge ranges cannot overflow an int
+  ka = fmin(fmax(ka, -4.0), (double)n + 4.0);
+  kb = fmin(fmax(kb, -4.0), (double)n + 4.0);
+  k0 = max(0, (int)floor(ka) - 1);
+  k1 = min(n - 1, (int)ceil(kb) + 1);
+}
+
+__global__ __launch_bounds__(kFwdT) void k_forward(KP P, const int *__restrict__ view_frame,
+                                                   const float4 *__restrict__ objt,
+                                                   float *__restrict__ out, int mode) {
+#pragma clang fp contract(off)
+  __shared__ float s_H[kHrStride];
+  const int ntile = (P.N_r + kFwdT - 1) / kFwdT;
+  const int nb = ntile * P.N_p;
+  int b = blockIdx.x;
+  if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+  const int v = b / ntile;
+  const int r = (b % ntile) * kFwdT + (int)threadIdx.x;
+  const int g = blockIdx.y, ngroups = gridDim.y;
+  const int z0 = g * kFwdZ;
+  const int nz = min(kFwdZ, P.N_z - z0);
+  const int f = view_frame[v];
+  float acc[kFwdZ];
+#pragma unroll
+  for (int q = 0; q < kFwdZ; q++) acc[q] = 0.0f;
+  if (f >= 0) {
+    for (int i = threadIdx.x; i < kHrStride; i += blockDim.x)
+      s_H[i] = P.hr[(size_t)v * kHrStride + i];
+    __syncthreads();
+  }
+  if (f >= 0 && r < P.N_r) {
+    const double c = P.cosv[v], s = P.sinv[v];
+    const float4 *ob = objt + (size_t)(f * ngroups + g) * P.N_x * P.N_y * 2;
+    // voxels whose projected centre x c - y s lies in [lo, hi) can cover bin r
+    const double lo = P.R0 + r * P.delta_r - P.delta_xy - 1e-9;
+    const double hi = P.R0 + (r + 1) * P.delta_r + P.delta_xy + 1e-9;
+    int y_first = 0, y_last = P.N_y - 1;
+    if (fabs(s) > 1e-6) {
+      // y s in (x c - hi, x c - lo] for some column centre x
+      const double xa = (P.x0 + 0.5 * P.delta_xy) * c;
+      const double xb = (P.x0 + (P.N_x - 0.5) * P.delta_xy) * c;
+      fwd_span(fmin(xa, xb) - hi, fmax(xa, xb) - lo, s, P.y0, P.delta_xy, P.N_y, y_first,
+               y_last);
+    }
+    for (int row = y_first; row <= y_last; row++) {
+      const double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
+      int k0 = 0, k1 = P.N_x - 1;
+      // x c in [lo + y s, hi + y s)
+      if (fabs(c) > 1e-6) fwd_span(lo + y * s, hi + y * s, c, P.x0, P.delta_xy, P.N_x, k0, k1);
+      for (int col = k0; col <= k1; col++) {
+        const float4 *px = ob + ((size_t)row * P.N_x + col) * 2;
+        const float4 p0 = px[0], p1 = px[1];
+        const double a = acol_bin(P, s_H, c, s, row, col, r);
+        if (a != 0.0) {
+          const float pv[kFwdZ] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+#pragma unroll
+          for (int q = 0; q < kFwdZ; q++) acc[q] = (float)((double)acc[q] + (double)pv[q] * a);
         }
       }
     }
+  }
+  if (r >= P.N_r) return;
+#pragma unroll
+  for (int q = 0; q < kFwdZ; q++) {
+    if (q >= nz) break;
+    const int z = z0 + q;
     for (int m = 0; m < P.mz; m++) {
       size_t o = sidx(P, z, v, r, m);
-      if (mode == 0) out[o] = acc;
-      else out[o] = (P.y[o] - acc) - P.d[(size_t)r * P.N_t + z * P.mz + m];
+      if (mode == 0) out[o] = acc[q];
+      else out[o] = (P.y[o] - acc[q]) - P.d[(size_t)r * P.N_t + z * P.mz + m];
     }
   }
 }
