@@ -98,9 +98,10 @@ def multires_schedule(proj_rows: int, proj_cols: int):
     return list(zip(mxy[::-1], mz[::-1]))
 
 
-def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4):
-    """Plain (zero) initialisation under reconstruct()'s multires schedule on
-    the GPU, every stage stopped by the reference's own rule (convergence
+def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4, init: str = "zero"):
+    """Plain initialisation (init "zero", the reference's, or "fbp": stage 0
+    from the filtered back-projection) under reconstruct()'s multires
+    schedule on the GPU, every stage stopped by the reference's own rule (convergence
     ratio below convg_thresh after iteration 1, XT_ICD_update.c:1066-1104 and
     :1188): `seconds` is the wall time from the start of the first stage to
     that stop on the finest stage, with the MAP cost evaluated every finest
@@ -124,6 +125,8 @@ def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4):
             st.init_from_stage(prev)
             st.set_offset(prev.offset())
             prev.close()
+        elif init == "fbp":
+            st.init_fbp()
         st.init_error_sino()
         finest = s == len(sched) - 1
         rule_it = None
@@ -160,7 +163,8 @@ def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4):
     out = dict(seconds=stop_t, finest_rule_stop_iteration=stop_it, stages=stages,
                finest_iterations_run=len(trace), cost_at_stop=trace[stop_it - 1][1] if stop_it
                else None, finest_cost_at={str(i): trace[i - 1][1] for i in marks},
-               schedule=[list(x) for x in sched], init="zero (plain)",
+               schedule=[list(x) for x in sched],
+               init="zero (plain)" if init == "zero" else "filtered back-projection at stage 0",
                rule="convergence ratio < convg_thresh (reference rule)")
     if finest_cap > 0:
         ccap = trace[-1][1]
@@ -276,6 +280,9 @@ def main():
         log("time to converged ...")
         line["time_to_converged"] = time_to_converged(
             scan, finest_cap=args.finest_cap if args.converge else 0)
+        log("time to converged from the filtered back-projection ...")
+        line["time_to_converged_fbp_init"] = time_to_converged(
+            scan, finest_cap=args.finest_cap if args.converge else 0, init="fbp")
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)

@@ -55,6 +55,9 @@ typedef struct {
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront
                                      per SIMD, else 1 or 2 */
+  int32_t init_fbp;               /* 1 = reconstruct() starts stage 0 from
+                                     tb_init_fbp instead of zero (default 0,
+                                     the reference's start) */
 } tb_options;
 
 void tb_get_options(tb_options *opt);
@@ -110,6 +113,13 @@ double tb_get_var_est(const tb_stage *st);
 int tb_init_error_sino(tb_stage *st);
 /* Forward projection alone (no y, no d): out = A x, [N_p][N_r][N_t]. */
 int tb_forward_project(tb_stage *st, float *out);
+/* Filtered back-projection of the stage's measurements into its object:
+ * y - d averaged over each slice's detector rows, Ram-Lak filter at the bin
+ * pitch, each frame from its own views (pi / views weight), linear
+ * interpolation, zero outside the mask.  No counterpart in the reference,
+ * whose stage 0 starts from OBJECT_INIT_VAL = 0 (XT_ICD_update.c:549-553);
+ * reconstruct() uses it for stage 0 when tb_options.init_fbp is set. */
+int tb_init_fbp(tb_stage *st);
 
 /* computeCost (XT_ICD_update.c:186-319), FP64 reductions. */
 int tb_cost(tb_stage *st, double *total, double *forward, double *prior);

@@ -135,3 +135,48 @@ class MapProblem:
                        options=dict(maxiter=iters, maxcor=50, ftol=1e-16, gtol=1e-12))
         full[m] = res.x
         return res.fun, full.reshape(self.shape)
+
+
+# --------------------------------------------------------------------------
+# filtered back-projection (the initial object of tb_init_fbp)
+# --------------------------------------------------------------------------
+
+def ramp_matrix(n_bins: int, h: float) -> np.ndarray:
+    """[n][n] matrix of the Ram-Lak filter sampled at pitch h (Kak and Slaney
+    eq. 3.61) times h: q = R @ p is the linear (zero-padded) convolution."""
+    n = np.arange(n_bins)[:, None] - np.arange(n_bins)[None, :]
+    k = np.zeros(n.shape)
+    k[n == 0] = 0.25
+    odd = (n % 2) != 0
+    k[odd] = -1.0 / (np.pi ** 2 * n[odd].astype(np.float64) ** 2)
+    return k / h
+
+
+def fbp(sino, angles, frame_views, n_x, n_y, x0, y0, dxy, r0, dr, mask=None) -> np.ndarray:
+    """sino [views][slices][bins] -> object [frames][slices][n_y][n_x]: each
+    frame back-projects its own views (frame_views[f], an index array) with
+    weight pi / len(views), the ray offset r = x cos - y sin of pixel centres
+    linearly interpolated between bin centres r0 + (j + 0.5) dr (zero
+    outside the detector); pixels outside mask are zero."""
+    sino = np.asarray(sino, np.float64)
+    q = np.einsum("rk,vzk->vzr", ramp_matrix(sino.shape[2], dr), sino)
+    x = x0 + (np.arange(n_x) + 0.5) * dxy
+    y = y0 + (np.arange(n_y) + 0.5) * dxy
+    X, Y = np.meshgrid(x, y)
+    nb = sino.shape[2]
+    qp = np.concatenate([np.zeros(q.shape[:2] + (1,)), q, np.zeros(q.shape[:2] + (1,))], axis=2)
+    out = np.zeros((len(frame_views), sino.shape[1], n_y, n_x))
+    for f, views in enumerate(frame_views):
+        for v in views:
+            th = float(angles[v])
+            u = (X * np.cos(th) - Y * np.sin(th) - r0) / dr - 0.5
+            j = np.floor(u)
+            t = u - j
+            j0 = np.clip(j.astype(np.int64) + 1, 0, nb + 1)
+            j1 = np.clip(j.astype(np.int64) + 2, 0, nb + 1)
+            out[f] += (1 - t)[None] * qp[v][:, j0] + t[None] * qp[v][:, j1]
+        if len(views):
+            out[f] *= np.pi / len(views)
+    if mask is not None:
+        out *= mask[None, None]
+    return out

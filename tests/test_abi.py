@@ -31,6 +31,26 @@ def test_library_exports_every_declared_entry_point():
     assert sorted(set(tb.EXPORTS)) == names
 
 
+def header_struct(name):
+    """(type, field) pairs of `typedef struct { ... } name;` in timbir_dev.h."""
+    text = open(os.path.join(ROOT, "include", "timbir_dev.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + ";", text).group(1)
+    return [tuple(m.groups()) for m in re.finditer(r"(\w+)\s+(\w+);", body)]
+
+
+@pytest.mark.parametrize("name,cls", [("tb_options", "TbOptions"), ("tb_stats", "TbStats")])
+def test_binding_structs_match_the_header(name, cls):
+    # the ctypes mirrors have the header's fields in the header's order and
+    # types, so the library and the binding agree on every offset
+    ctype = {"int32_t": C.c_int32, "uint32_t": C.c_uint32, "int64_t": C.c_int64,
+             "double": C.c_double}
+    fields = header_struct(name)
+    got = getattr(tb, cls)._fields_
+    assert [f for _, f in fields] == [f for f, _ in got]
+    assert [ctype[t] for t, _ in fields] == [t for _, t in got]
+
+
 def test_options_round_trip_without_gpu():
     o = tb.get_options()
     assert o.max_iterations == 1000

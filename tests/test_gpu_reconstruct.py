@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 # the library's defaults (tb_get_options before any change)
 DEFAULTS = dict(compute_cost=0, abort_on_cost_increase=0, max_iterations=1000, no_nhicd=0,
-                write_files=1, order_seed=0, verbose=1, kernel=0, views_per_lane=0)
+                write_files=1, order_seed=0, verbose=1, kernel=0, views_per_lane=0,
+                init_fbp=0)
 
 
 def log(msg):

@@ -21,7 +21,7 @@ EXPORTS = [
     "tb_stage_destroy", "tb_stage_info", "tb_set_object", "tb_get_object", "tb_set_offset",
     "tb_get_offset", "tb_set_magmap", "tb_get_magmap", "tb_init_from_coarse", "tb_init_from_stage",
     "tb_get_error_sino", "tb_get_weights", "tb_get_detector_response", "tb_get_var_est",
-    "tb_init_error_sino", "tb_forward_project", "tb_cost", "tb_offset_update",
+    "tb_init_error_sino", "tb_forward_project", "tb_init_fbp", "tb_cost", "tb_offset_update",
     "tb_variance_update", "tb_voxel_update", "tb_update_colour", "tb_vsc_select",
     "tb_nhicd_list", "tb_icd_iteration", "tb_stage_run", "tb_get_stats", "tb_reset_stats",
     "tb_synchronize",
@@ -32,7 +32,8 @@ class TbOptions(C.Structure):
     _fields_ = [("compute_cost", C.c_int32), ("abort_on_cost_increase", C.c_int32),
                 ("max_iterations", C.c_int32), ("no_nhicd", C.c_int32),
                 ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32),
-                ("kernel", C.c_int32), ("views_per_lane", C.c_int32)]
+                ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
+                ("init_fbp", C.c_int32)]
 
 
 class TbStats(C.Structure):
@@ -216,6 +217,9 @@ class Stage:
 
     def init_from_stage(self, prev: "Stage"):
         _check(self.L.tb_init_from_stage(self.h, prev.h))
+
+    def init_fbp(self):
+        _check(self.L.tb_init_fbp(self.h))
 
     def init_error_sino(self):
         _check(self.L.tb_init_error_sino(self.h))

@@ -19,6 +19,8 @@ namespace tb {
 __global__ void k_transpose_in(KP, const float *, const float *, double);
 __global__ void k_to_ref_layout(KP, const float *, float *);
 __global__ void k_fwd_interleave(KP, float4 *);
+__global__ void k_fbp_filter(KP, float *);
+__global__ void k_fbp_back(KP, const float *);
 __global__ void k_forward(KP, const int *, const float4 *, float *, int);
 __global__ void k_upsample(const float *, float *, int, int, int, int, int);
 template <int NT>
@@ -56,7 +58,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -733,6 +735,22 @@ static int run_forward(tb_stage *st, float *out, int mode) {
 
 extern "C" int tb_init_error_sino(tb_stage *st) {
   if (run_forward(st, st->e, 1)) return -1;
+  return 0;
+}
+
+extern "C" int tb_init_fbp(tb_stage *st) {
+  if ((size_t)st->N_r * sizeof(double) > 65536)
+    return fail("tb_init_fbp: %d detector bins exceed the filter's LDS row", st->N_r);
+  KP P = st->kp();
+  float *q = nullptr;
+  HIPCHK(dalloc(&q, (size_t)st->N_z * st->N_p * st->N_r));
+  hipLaunchKernelGGL(k_fbp_filter, dim3(st->N_p, st->N_z), dim3(256),
+                     st->N_r * sizeof(double), st->stream, P, q);
+  hipLaunchKernelGGL(k_fbp_back, dim3(grid_for(st->nobj())), dim3(256), 0, st->stream, P, q);
+  hipError_t err = hipGetLastError();
+  if (err == hipSuccess) err = hipStreamSynchronize(st->stream);
+  (void)hipFree(q);
+  HIPCHK(err);
   return 0;
 }
 

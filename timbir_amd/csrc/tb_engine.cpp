@@ -237,6 +237,9 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
       // restart: the previous stage's files
       rc = tb_init_from_coarse(st, prev_obj.data(), prev_nz, prev_mag.data());
       if (!rc && !prev_off.empty()) rc = tb_set_offset(st, prev_off.data());
+    } else if (opt.init_fbp) {
+      // stage 0 from the filtered back-projection instead of zero
+      rc = tb_init_fbp(st);
     }
     int iters = 0;
     std::vector<double> costs;

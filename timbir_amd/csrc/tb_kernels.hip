@@ -231,6 +231,75 @@ __global__ __launch_bounds__(kFwdT) void k_forward(KP P, const int *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Filtered back-projection (an initial object; the reference starts every
+// reconstruction from zero, OBJECT_INIT_VAL in XT_Constants.h).  Parallel
+// beam, ray offset r = x cos(theta) - y sin(theta) as the A matrix uses.
+//   k_fbp_filter: q[z][v][r] = ramp * p, p the slice's measurements y - d
+//     averaged over its mz detector rows; the Ram-Lak kernel sampled at the
+//     bin pitch h (Kak and Slaney, eq. 3.61): 1/(4h^2) at 0, -1/(n pi h)^2
+//     at odd n, 0 at even n, times h.  Linear convolution (zero outside the
+//     detector), FP64 sums in a fixed order.
+//   k_fbp_back: x[f][z][pixel] = (pi / n_f) sum over the n_f views of frame
+//     f of q at the pixel's ray, linearly interpolated between bin centres;
+//     zero outside the reconstruction mask.
+// ---------------------------------------------------------------------------
+__global__ void k_fbp_filter(KP P, float *__restrict__ q) {
+  extern __shared__ double s_p[];  // one detector row
+  const int v = blockIdx.x, z = blockIdx.y;
+  for (int r = threadIdx.x; r < P.N_r; r += blockDim.x) {
+    double acc = 0.0;
+    for (int m = 0; m < P.mz; m++) {
+      const size_t o = sidx(P, z, v, r, m);
+      acc += (double)P.y[o] - (double)P.d[(size_t)r * P.N_t + z * P.mz + m];
+    }
+    s_p[r] = acc / P.mz;
+  }
+  __syncthreads();
+  const double h = P.delta_r;
+  const double k_odd = -1.0 / (kPi * kPi);
+  for (int r = threadIdx.x; r < P.N_r; r += blockDim.x) {
+    double acc = 0.25 * s_p[r];
+    // odd offsets n = r - k
+    for (int k = (r & 1) ^ 1; k < P.N_r; k += 2) {
+      const double n = (double)(r - k);
+      acc += k_odd * s_p[k] / (n * n);
+    }
+    q[((size_t)z * P.N_p + v) * P.N_r + r] = (float)(acc / h);
+  }
+}
+
+__global__ void k_fbp_back(KP P, const float *__restrict__ q) {
+  const size_t Nxy = (size_t)P.N_x * P.N_y;
+  const size_t n = (size_t)P.N_time * P.N_z * Nxy;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t pix = i % Nxy;
+    const int z = (int)((i / Nxy) % P.N_z), f = (int)(i / (Nxy * P.N_z));
+    const int row = (int)(pix / P.N_x), col = (int)(pix % P.N_x);
+    float out = 0.0f;
+    const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+    if (P.mask[pix] && nv > 0) {
+      const double x = P.x0 + ((double)col + 0.5) * P.delta_xy;
+      const double y = P.y0 + ((double)row + 0.5) * P.delta_xy;
+      double acc = 0.0;
+      for (int k = 0; k < nv; k++) {
+        const int v = P.view_idx[vb + k];
+        const double u = (x * P.cosv[v] - y * P.sinv[v] - P.R0) / P.delta_r - 0.5;
+        const double j = floor(u);
+        const double t = u - j;
+        const int j0 = (int)j;
+        const float *qv = q + ((size_t)z * P.N_p + v) * P.N_r;
+        const double q0 = (j0 >= 0 && j0 < P.N_r) ? (double)qv[j0] : 0.0;
+        const double q1 = (j0 + 1 >= 0 && j0 + 1 < P.N_r) ? (double)qv[j0 + 1] : 0.0;
+        acc += (1.0 - t) * q0 + t * q1;
+      }
+      out = (float)(acc * kPi / nv);
+    }
+    P.obj[i] = out;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Voxel update: one workgroup per cell of one colour, voxels in sequence.
 // Per voxel (compute_voxel_update_Atten, XT_VoxUpdate.c:43-115):
 //   threads stride over the frame's views, build the A column on the fly and
