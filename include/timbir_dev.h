@@ -43,14 +43,17 @@ typedef struct {
   int32_t kernel;                 /* voxel-update kernel: 0 = the fastest
                                      that applies (LDS-resident multi-
                                      wavefront, then multi-wavefront with e
-                                     in global memory, then one-wavefront,
+                                     in global memory, then wide-column
+                                     multi-wavefront, then one-wavefront,
                                      then one-voxel), 1 = the one-voxel
                                      kernel only, 2 = the one-wavefront LDS
                                      kernel where it fits, 3 = the
                                      multi-wavefront LDS kernel where it
                                      fits, 4 = the multi-wavefront kernel
                                      with e in global memory where it
-                                     applies */
+                                     applies, 5 = the wide-column
+                                     multi-wavefront kernel (columns of
+                                     up to 17 bins) where it applies */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront

@@ -34,13 +34,13 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1, 2, 3, 4],
-                ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal"])
+@pytest.fixture(params=[0, 1, 2, 3, 4, 5],
+                ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal", "wide"])
 def kernel(request):
     """Voxel-update kernel: 0 = the fastest that applies, 1 = the one-voxel
     kernel everywhere, 2 = the one-wavefront LDS kernel, 3 = the
     multi-wavefront LDS kernel, 4 = the multi-wavefront kernel with e in
-    global memory."""
+    global memory, 5 = the wide-column multi-wavefront kernel."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -55,7 +55,7 @@ def check_kernel(g, kernel):
         assert st["lds_launches"] + st["global_launches"] == st["voxel_launches"], st
     elif kernel in (2, 3):
         assert st["lds_launches"] == st["voxel_launches"], st
-    elif kernel == 4:
+    elif kernel in (4, 5):
         assert st["global_launches"] == st["voxel_launches"], st
     else:
         assert st["lds_launches"] == 0 and st["global_launches"] == 0, st
@@ -276,6 +276,8 @@ def test_icd_iterations_track_oracle(quad, rings, kernel):
 def test_converges_to_map(kernel):
     """GPU ICD reaches the scipy minimiser of the explicit MAP cost."""
     import mathref
+    if kernel == 5:
+        pytest.skip("the wide-column kernel needs at least 17 detector bins")
     log(f"converge to MAP kernel={kernel}")
     scan = scan_small(N=12, nz=4, n_views=16)
     o = pyorc.Problem(scan, opt=pyorc.options(), precision="f64")
@@ -344,15 +346,19 @@ def test_phases_track_oracle(kernel, c=1.0):
     (4, 4, 128, 2),   # 64 views per frame: four views per lane, three theta waves
     (4, 4, 200, 2),   # 100 views per frame: 15 theta waves
     (4, 4, 300, 2),   # 150 views per frame: two views per lane
+    (8, 8, 64, 2),    # 17 bins x 8 rows per view: one row per lane, 4 theta waves
+    (8, 8, 256, 2),   # 128 views per frame: two rows per lane, 8 theta waves
 ])
 def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
     """Every instantiation of the voxel-update kernels follows the oracle."""
-    if mxy == 4 and kernel in (2, 3):
+    if mxy >= 4 and kernel in (2, 3):
         pytest.skip("the LDS kernels take columns of at most 5 bins x 2 rows")
+    if mxy == 8 and kernel == 4:
+        pytest.skip("the column table holds at most 9 bins")
     log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
-    n, nz = (64, 8) if mxy == 4 or n_views == 128 else (32, 4)
+    n, nz = (64, 16) if mxy == 8 else (64, 8) if mxy == 4 or n_views == 128 else (32, 4)
     scan = scan_small(N=n, nz=nz, n_views=n_views, n_frames=n_frames, c_s=1.0, c_t=1.0)
-    stage = 1 if mxy > 1 else 0
+    stage = 1 if 1 < mxy < 8 else 0
     o, g = pair(scan, stage_idx=stage, mult_xy=mxy, mult_z=mz,
                 prev_mult_z=2 * mz if stage else 0)
     x = random_object(o.obj_shape, o.mask(), seed=5)

@@ -31,6 +31,8 @@ template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mw(KP, const int2 *, int, int, int, int, int);
 template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
+template <int NBX, int R, int MAXTH>
+__global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
 template <int NB>
 __global__ void k_build_columns(KP);
 __global__ void k_offset_partial(KP, int, int, double *, double *);
@@ -971,6 +973,26 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
         fn = vpl == 1 ? k_icd_mwg<9, 4, 1, 15> : k_icd_mwg<9, 4, 2, 15>;
     }
     if (fn && run_mw(fn, gnb, nth, vpw, 0, true) < 0) return -1;
+  }
+  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5) && 2 * st->mult_xy + 1 <= 17 &&
+      st->N_r >= 17) {
+    // columns of up to 17 bins (the coarsest stages): a lane per (view, R
+    // detector rows), the fewest rows per lane that fit in 8 theta waves
+    const int mz = st->mult_z;
+    for (int R = 1; R <= 2 && !launched; R++) {
+      if (mz % R != 0 || st->max_nv * (mz / R) > 8 * 64) continue;
+      const int nth = (st->max_nv * (mz / R) + 63) / 64;
+      const size_t lds = (size_t)st->max_nv * kHrStride * sizeof(float);
+      if (lds + 2048 > st->lds_max) break;
+      mw_fn fn = R == 1 ? k_icd_wide<17, 1, 8> : k_icd_wide<17, 2, 8>;
+      if (seq == 0 && ensure_order(st, iter)) return -1;
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
+                         chunk, seq, count_mag, mz / R);
+      st->global_launches++;
+      launched = true;
+    }
   }
 
 

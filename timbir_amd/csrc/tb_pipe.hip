@@ -1220,6 +1220,156 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *
   }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-wavefront voxel update for wide columns: the coarsest multires
+// stages, whose columns (up to NBX bins x the slice's mz rows, e.g. 17 bins x
+// 8 rows at mult 8) exceed the column table's 9 bins.  A lane owns one view
+// of the cell and R of the slice's detector rows, for every bin: it computes
+// the voxel's column in that view (col_u / col_window / col_value, the
+// arithmetic of the column table), reads the NBX x R (e, w) entries of its
+// window, adds its theta terms, and after the exchange writes its own
+// entries back, so each entry is read and written by one lane only and the
+// sequential sweep follows from program order.  The prior wave, the
+// exchange and the step are those of k_icd_mw.  The views' detector
+// responses sit in LDS.
+// ---------------------------------------------------------------------------
+template <int R>
+__device__ __forceinline__ void ld_r(const float *p, float (&o)[R]) {
+  if constexpr (R == 1) {
+    o[0] = p[0];
+  } else {
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+  }
+}
+
+template <int NBX, int R, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wide(KP P, const int2 *__restrict__ cells,
+                                                            int iter, int chunk, int seq,
+                                                            int count_mag, int ngr) {
+  __shared__ StepIn S;
+  extern __shared__ float s_H[];  // [views of the frame][kHrStride]
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0 || nv <= 0) return;
+  for (int i = threadIdx.x; i < nv * kHrStride; i += blockDim.x)
+    s_H[i] = P.hr[(size_t)P.view_idx[vb + i / kHrStride] * kHrStride + i % kHrStride];
+  if (threadIdx.x < 32) {
+    (&S.t1[0][0])[threadIdx.x] = 0.0;
+    (&S.t2[0][0])[threadIdx.x] = 0.0f;
+  }
+  __syncthreads();
+
+  if (wave < nth) {
+    // ---------------- theta wave ----------------
+    const int t = wave * 64 + lane;
+    const int q = t / ngr, g = t - q * ngr;
+    const bool act = q < nv;
+    const int v = P.view_idx[vb + (act ? q : 0)];
+    double Au, Bu, Cu;
+    col_coef(P, v, Au, Bu, Cu);
+    const float *H = s_H + (act ? q : 0) * kHrStride;
+    const float *eb = P.e + sidx(P, z, v, 0, g * R);
+    const float *wb = P.w + sidx(P, z, v, 0, g * R);
+    float *ebw = P.e + sidx(P, z, v, 0, g * R);
+    const int *lst = list + n0;
+    unsigned long long nent = 0;
+    int pix_next = lst[0];
+    for (int n = 0; n < cnt; n++) {
+      const int par = n & 1;
+      const int pix = pix_next;
+      pix_next = lst[min(n + 1, cnt - 1)];
+      const int yy = pix >> 16, xx = pix & 0xffff;
+      const double u = col_u(xx, yy, Au, Bu, Cu);
+      int loc, hic;
+      const int w0 = act ? col_window(P, u, NBX, loc, hic) : -1;
+      const int ws = max(w0, 0);
+      float e[NBX][R], w[NBX][R], a[NBX];
+#pragma unroll
+      for (int j = 0; j < NBX; j++) {
+        ld_r<R>(eb + (size_t)(ws + j) * P.mz, e[j]);
+        ld_r<R>(wb + (size_t)(ws + j) * P.mz, w[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < NBX; j++) a[j] = w0 >= 0 ? col_value(P, H, u, ws + j, loc, hic) : 0.0f;
+      // theta sums (XT_VoxUpdate.c:61-92) with the generalized-Huber weights
+      // (XT_VoxUpdate.c:66-80)
+      double t1 = 0.0;
+      float t2 = 0.0f;
+#pragma unroll
+      for (int j = 0; j < NBX; j++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const float ee = e[j][r], ww = w[j][r];
+          const float cw = huber_sel(P, ee, ww)
+                               ? ww
+                               : P.Td * __builtin_amdgcn_sqrtf(ww) * __builtin_amdgcn_rcpf(fabsf(ee));
+          const float ac = a[j] * cw;
+          t1 = t1 + (double)(-ee * ac);
+          t2 = t2 + a[j] * ac;
+        }
+        nent += a[j] > 0.0f ? R : 0;
+      }
+      wave_total_df(t1, t2);
+      if (lane == 0) {
+        S.t1[par][wave] = t1;
+        S.t2[par][wave] = t2;
+      }
+      __syncthreads();
+      float V;
+      bool skip;
+      const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
+      const float dx = xn - V;
+      if (dx != 0.0f) {
+#pragma unroll
+        for (int j = 0; j < NBX; j++) {
+          if (a[j] != 0.0f) {
+            float o[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) o[r] = fmaf(-a[j], dx, e[j][r]);
+            if constexpr (R == 1) ebw[(size_t)(ws + j) * P.mz] = o[0];
+            else *reinterpret_cast<float2 *>(ebw + (size_t)(ws + j) * P.mz) = make_float2(o[0], o[1]);
+          }
+        }
+      }
+    }
+    const double nsum = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
+  } else {
+    // ---------------- prior wave ----------------
+    prior_wave<MAXTH>(P, S, list, n0, cnt, f, z, iter, seq, count_mag, nth, lane);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    P.cell_ent[cell] += ne;
+  }
+}
+
+// up to 8 theta waves: with more, the 17 x R (e, w) registers spill
+template __global__ void k_icd_wide<17, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wide<17, 2, 8>(KP, const int2 *, int, int, int, int, int);
+
 template __global__ void k_icd_mwg<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 1, 8>(KP, const int2 *, int, int, int, int, int);
