@@ -346,6 +346,10 @@ def test_phases_track_oracle(kernel, c=1.0):
     (4, 4, 128, 2),   # 64 views per frame: four views per lane, three theta waves
     (4, 4, 200, 2),   # 100 views per frame: 15 theta waves
     (4, 4, 300, 2),   # 150 views per frame: two views per lane
+    (1, 1, 1200, 2),  # 600 views per frame: four views per lane, 8 theta waves
+    (2, 2, 800, 2),   # 400 views per frame: four views per lane, 9 theta waves
+    (4, 2, 160, 2),   # 9 bins x 2 rows per view: one view per lane, 12 theta waves
+    (4, 2, 400, 2),   # 200 views per frame: two views per lane
     (8, 8, 64, 2),    # 17 bins x 8 rows per view: one row per lane, 4 theta waves
     (8, 8, 256, 2),   # 128 views per frame: two rows per lane, 8 theta waves
 ])
@@ -355,8 +359,14 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
         pytest.skip("the LDS kernels take columns of at most 5 bins x 2 rows")
     if mxy == 8 and kernel == 4:
         pytest.skip("the column table holds at most 9 bins")
+    nv = n_views // n_frames
+    if nv > 256 and kernel in (2, 3):
+        pytest.skip("the cell's (e, w) block does not fit in LDS")
+    if kernel == 5 and nv * (mz // min(mz, 2)) > 8 * 64:
+        pytest.skip("more views x rows than 8 theta waves of the wide-column kernel hold")
     log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
-    n, nz = (64, 16) if mxy == 8 else (64, 8) if mxy == 4 or n_views == 128 else (32, 4)
+    n, nz = ((64, 16) if mxy == 8 else (64, 8) if mxy == 4 or n_views == 128
+             else (32, 4))
     scan = scan_small(N=n, nz=nz, n_views=n_views, n_frames=n_frames, c_s=1.0, c_t=1.0)
     stage = 1 if 1 < mxy < 8 else 0
     o, g = pair(scan, stage_idx=stage, mult_xy=mxy, mult_z=mz,

@@ -958,19 +958,35 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     const int w = 2 * st->mult_xy + 1;
     const int gnb = w <= 3 ? 3 : w <= 5 ? 5 : w <= 9 ? 9 : 0;
     const int mz = st->mult_z;
+    // instantiations: (NB, MZ, VPL) -> the widest MAXTH that keeps the
+    // registers unspilled (15, or 11 with four views per lane)
+    auto pick = [&](int vpl, int nth) -> mw_fn {
+      if (gnb == 3 && mz == 1) {
+        if (vpl == 1) return nth <= 8 ? k_icd_mwg<3, 1, 1, 8> : k_icd_mwg<3, 1, 1, 15>;
+        if (vpl == 2) return k_icd_mwg<3, 1, 2, 15>;
+        if (vpl == 4 && nth <= 11) return k_icd_mwg<3, 1, 4, 11>;
+      } else if (gnb == 5 && mz == 2) {
+        if (vpl == 1) return nth <= 8 ? k_icd_mwg<5, 2, 1, 8> : k_icd_mwg<5, 2, 1, 15>;
+        if (vpl == 2) return k_icd_mwg<5, 2, 2, 15>;
+        if (vpl == 4 && nth <= 11) return k_icd_mwg<5, 2, 4, 11>;
+      } else if (gnb == 9 && mz == 4) {
+        if (vpl == 1) return k_icd_mwg<9, 4, 1, 15>;
+        if (vpl == 2) return k_icd_mwg<9, 4, 2, 15>;
+      } else if (gnb == 9 && mz == 2) {
+        if (vpl == 1) return k_icd_mwg<9, 2, 1, 15>;
+        if (vpl == 2) return k_icd_mwg<9, 2, 2, 15>;
+        if (vpl == 4 && nth <= 11) return k_icd_mwg<9, 2, 4, 11>;
+      }
+      return nullptr;
+    };
     mw_fn fn = nullptr;
     int nth = 0, vpw = 0;
-    for (int vpl = 1; vpl <= 2 && !fn && gnb && st->N_r >= gnb; vpl++) {
+    for (int vpl = 1; vpl <= 4 && !fn && gnb && st->N_r >= gnb; vpl *= 2) {
       const int cap = (64 / gnb) * vpl;
       nth = (st->max_nv + cap - 1) / cap;
       vpw = (st->max_nv + nth - 1) / nth;
       if (nth > 15) continue;
-      if (gnb == 3 && mz == 1 && vpl == 1)
-        fn = nth <= 8 ? k_icd_mwg<3, 1, 1, 8> : k_icd_mwg<3, 1, 1, 15>;
-      else if (gnb == 5 && mz == 2 && vpl == 1)
-        fn = nth <= 8 ? k_icd_mwg<5, 2, 1, 8> : k_icd_mwg<5, 2, 1, 15>;
-      else if (gnb == 9 && mz == 4)
-        fn = vpl == 1 ? k_icd_mwg<9, 4, 1, 15> : k_icd_mwg<9, 4, 2, 15>;
+      fn = pick(vpl, nth);
     }
     if (fn && run_mw(fn, gnb, nth, vpw, 0, true) < 0) return -1;
   }

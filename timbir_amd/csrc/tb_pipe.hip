@@ -1376,6 +1376,15 @@ template __global__ void k_icd_mwg<5, 2, 1, 8>(KP, const int2 *, int, int, int, 
 template __global__ void k_icd_mwg<5, 2, 1, 15>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<9, 4, 1, 15>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<9, 4, 2, 15>(KP, const int2 *, int, int, int, int, int);
+// more views per lane for cells with many views (single-frame scans); with
+// 4 views per lane the registers fit only 11 theta waves without spilling
+template __global__ void k_icd_mwg<3, 1, 2, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 4, 11>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 2, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 4, 11>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 2, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 2, 2, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 2, 4, 11>(KP, const int2 *, int, int, int, int, int);
 
 // finest stage (m_xy = m_z = 1: at most 3 bins per view, one detector row)
 template __global__ void k_icd_lds<1, 3, 1, 4>(KP, const int2 *, int, int, int, int);
