@@ -81,23 +81,6 @@ def cpu_baseline(scan, stage_kw, threads: int, budget_s: float = 20.0):
                        f"{upd} voxel updates in {el:.1f} s")
 
 
-def multires_schedule(proj_rows: int, proj_cols: int):
-    """(mult_xy, mult_z) per stage, coarse to fine, as reconstruct() picks
-    them (XT_Engine.c:111-141 with one node)."""
-    import math
-    nres = int(math.log(proj_cols / 64.0) / math.log(2.0) + 1)
-    nres = max(2, min(8, nres))
-    mxy, mz = [1], [1]
-    for i in range(1, nres):
-        if proj_cols % (mxy[-1] * 2) != 0:
-            nres = i
-            break
-        mxy.append(mxy[-1] * 2)
-        mz.append(mz[-1] * 2 if proj_rows % (mz[-1] * 2) == 0 and proj_rows // (mz[-1] * 2) >= 2
-                  else mz[-1])
-    return list(zip(mxy[::-1], mz[::-1]))
-
-
 def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4, init: str = "zero"):
     """Plain initialisation (init "zero", the reference's, or "fbp": stage 0
     from the filtered back-projection) under reconstruct()'s multires
@@ -109,6 +92,7 @@ def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4, init: str = 
     finest_cap iterations so that the cost trajectory shows how far that stop
     is from the MAP point: `cost_gap_at_stop` = (C_stop - C_cap) / |C_cap|."""
     import timbir_amd as tb
+    from timbir_amd.batch import multires_schedule
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     t0 = time.perf_counter()
     prev = None

@@ -1,0 +1,75 @@
+"""Many independent reconstructions at once.
+
+reconstruct() (src/include/mbir4d.h:34) takes one scan.  A small scan keeps
+only a few CUs busy: its voxel kernels run one workgroup per cell (frame,
+slice) of one colour, and a single-frame scan with 4 slices has 2 cells per
+colour.  reconstruct_many() runs the same stage pipeline for many scans from
+a pool of host threads; every stage owns a HIP stream, so the kernels of
+different scans overlap on the GPU.  Each result equals, bit for bit, what
+reconstruct() returns for that scan alone (no files are written).
+
+How many streams the GPU runs side by side is capped by the hardware queues
+HIP gives the process: GPU_MAX_HW_QUEUES (4 unless set before the first HIP
+call) bounds the useful number of workers.
+"""
+from __future__ import annotations
+
+import math
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+MIN_XY_RECON_RES = 64.0  # XT_Constants.h:99
+
+
+def multires_schedule(proj_rows: int, proj_cols: int) -> list:
+    """(mult_xy, mult_z) per stage, coarse to fine, as reconstruct() picks
+    them (XT_Engine.c:111-141 with one node)."""
+    nres = int(math.log(proj_cols / MIN_XY_RECON_RES) / math.log(2.0) + 1)
+    nres = max(2, min(8, nres))
+    mxy, mz = [1], [1]
+    for i in range(1, nres):
+        if proj_cols % (mxy[-1] * 2) != 0:
+            break
+        mxy.append(mxy[-1] * 2)
+        mz.append(mz[-1] * 2 if proj_rows % (mz[-1] * 2) == 0 and proj_rows // (mz[-1] * 2) >= 2
+                  else mz[-1])
+    return list(zip(mxy[::-1], mz[::-1]))
+
+
+def reconstruct_stages(scan: dict) -> np.ndarray:
+    """reconstruct()'s schedule through the stage API, without files: each
+    stage from the previous one in HBM (XT_Engine.c:156-182), run to the
+    reference's stop rule.  Returns the finest object
+    [recon_num][rows][cols][cols]."""
+    import timbir_amd as tb
+    sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
+    prev = None
+    var_est = 1.0
+    st = None
+    try:
+        for s, (mxy, mz) in enumerate(sched):
+            st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
+                          prev_mult_z=sched[s - 1][1] if s else 0, var_est=var_est)
+            if prev is not None:
+                st.init_from_stage(prev)
+                st.set_offset(prev.offset())
+                prev.close()
+                prev = None
+            st.stage_run(0, with_cost=False)
+            if s < len(sched) - 1:
+                prev, var_est, st = st, st.var_est, None
+        return st.object()
+    finally:
+        for h in (prev, st):
+            if h is not None:
+                h.close()
+
+
+def reconstruct_many(scans, workers: int = 4) -> list:
+    """reconstruct_stages() for every scan, `workers` scans at a time, each
+    in its own host thread with its own streams.  Results in input order."""
+    if workers <= 1:
+        return [reconstruct_stages(s) for s in scans]
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        return list(pool.map(reconstruct_stages, scans))
