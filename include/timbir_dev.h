@@ -52,8 +52,9 @@ typedef struct {
                                      fits, 4 = the multi-wavefront kernel
                                      with e in global memory where it
                                      applies, 5 = the wide-column
-                                     multi-wavefront kernel (columns of
-                                     up to 17 bins) where it applies */
+                                     multi-wavefront kernels (columns of
+                                     up to 17 bins, views in passes when
+                                     there are many) where they apply */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront

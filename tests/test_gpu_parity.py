@@ -176,6 +176,8 @@ def test_cost_matches_oracle(quad):
 
 
 def test_single_voxel_update(kernel):
+    if kernel == 5:
+        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
     log(f"single voxel kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40)
     o, g = pair(scan)
@@ -254,6 +256,8 @@ def test_vsc_select_lists():
 def test_icd_iterations_track_oracle(quad, rings, kernel):
     """Same colour schedule on both sides: the trajectories agree to FP32
     (non-stiff prior, see test_phases_track_oracle)."""
+    if kernel == 5:
+        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
     log(f"icd trajectory quad={quad} rings={rings} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, quad_convex=quad, rings=0.01 if rings else 0.0,
                       zingers=1e-3 if not quad else 0.0, remove_rings=rings, c_s=1.0, c_t=1.0)
@@ -300,6 +304,8 @@ def test_phases_track_oracle(kernel, c=1.0):
     chaotic: two CPU oracles that differ only in float vs double arrays part
     by ~1% RMSE at iteration 2.  Trajectory parity is therefore checked with
     c = 1; stiff problems are checked at convergence (test_converges_to_map)."""
+    if kernel == 5:
+        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
     log(f"phase by phase c={c} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, c_s=c, c_t=c)
     o, g = pair(scan)
@@ -352,6 +358,8 @@ def test_phases_track_oracle(kernel, c=1.0):
     (4, 2, 400, 2),   # 200 views per frame: two views per lane
     (8, 8, 64, 2),    # 17 bins x 8 rows per view: one row per lane, 4 theta waves
     (8, 8, 256, 2),   # 128 views per frame: two rows per lane, 8 theta waves
+    (2, 2, 1200, 2),  # 600 views per frame: views walked in passes (k_icd_wpass)
+    (8, 8, 320, 2),   # 160 views per frame x 8 rows: passes of 17 bins x 2 rows
 ])
 def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
     """Every instantiation of the voxel-update kernels follows the oracle."""
@@ -360,10 +368,13 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
     if mxy == 8 and kernel == 4:
         pytest.skip("the column table holds at most 9 bins")
     nv = n_views // n_frames
+    mwg_views = {1: 924, 2: 528, 4: 210}.get(mxy, 0)
+    if kernel == 4 and nv > mwg_views:
+        pytest.skip("more views per cell than the global-e kernel's lanes hold")
     if nv > 256 and kernel in (2, 3):
         pytest.skip("the cell's (e, w) block does not fit in LDS")
-    if kernel == 5 and nv * (mz // min(mz, 2)) > 8 * 64:
-        pytest.skip("more views x rows than 8 theta waves of the wide-column kernel hold")
+    if kernel == 5 and mz == 1:
+        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
     log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
     n, nz = ((64, 16) if mxy == 8 else (64, 8) if mxy == 4 or n_views == 128
              else (32, 4))

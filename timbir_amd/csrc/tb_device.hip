@@ -33,6 +33,8 @@ template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int MAXTH>
 __global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
+template <int NBX, int R, int PMAX, int MAXTH>
+__global__ void k_icd_wpass(KP, const int2 *, int, int, int, int, int);
 template <int NB>
 __global__ void k_build_columns(KP);
 __global__ void k_offset_partial(KP, int, int, double *, double *);
@@ -1006,6 +1008,30 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
                          chunk, seq, count_mag, mz / R);
+      st->global_launches++;
+      launched = true;
+    }
+  }
+  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5) && st->mult_z % 2 == 0) {
+    // too many views for a lane per view: 11 theta waves own blocks of views
+    // and walk their (view, bin, 2 rows) entries in passes
+    const int w = 2 * st->mult_xy + 1;
+    const int nbx = w <= 5 ? 5 : w <= 9 ? 9 : w <= 17 ? 17 : 0;
+    constexpr int nth = 11;
+    const int vpw = (st->max_nv + nth - 1) / nth;
+    const int passes = (vpw * nbx * (st->mult_z / 2) + 63) / 64;
+    const size_t lds = (size_t)st->max_nv * 32;  // per-view coefficients
+    mw_fn fn = nullptr;
+    if (nbx == 17)
+      fn = passes <= 16 ? k_icd_wpass<17, 2, 16, 11> : passes <= 24 ? k_icd_wpass<17, 2, 24, 11> : nullptr;
+    else if (nbx == 9 && passes <= 16) fn = k_icd_wpass<9, 2, 16, 11>;
+    else if (nbx == 5 && passes <= 16) fn = k_icd_wpass<5, 2, 16, 11>;
+    if (fn && st->N_r >= nbx && lds + 4096 <= st->lds_max) {
+      if (seq == 0 && ensure_order(st, iter)) return -1;
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
+                         chunk, seq, count_mag, vpw);
       st->global_launches++;
       launched = true;
     }

@@ -1370,6 +1370,176 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wide(KP P, const int2 
 template __global__ void k_icd_wide<17, 1, 8>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_wide<17, 2, 8>(KP, const int2 *, int, int, int, int, int);
 
+// ---------------------------------------------------------------------------
+// Multi-wavefront voxel update for wide columns and many views: the coarse
+// multires stages whose cells neither the column-table kernels nor
+// k_icd_wide can hold (hundreds of views per cell).  Theta wave w owns a
+// contiguous block of the cell's views and is the only wave that reads or
+// writes their e rows, so the sequential sweep follows from program order,
+// as in k_icd_mwg.  Per voxel the wave walks its views' entries in passes
+// of 64 lanes: a lane takes one (view, bin of an NBX-bin window, R detector
+// rows) and consecutive lanes cover consecutive bins and rows of a view, so
+// every (e, w) access of a pass is contiguous per view.  The lane evaluates
+// its A value with the column-table arithmetic (col_u / col_window /
+// col_value) from per-view coefficients in LDS.  The prior wave, the
+// exchange and the step are those of k_icd_mw.
+// ---------------------------------------------------------------------------
+struct WideView {
+  double Au, Bu, Cu;
+  int v;
+  int pad;
+};
+
+template <int NBX, int R, int PMAX, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2 *__restrict__ cells,
+                                                            int iter, int chunk, int seq,
+                                                            int count_mag, int vpw) {
+  __shared__ StepIn S;
+  extern __shared__ WideView s_view[];  // [views of the frame]
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+  const int ngr = P.mz / R;      // row groups of a view
+  const int EPV = NBX * ngr;     // lane entries of a view
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0 || nv <= 0) return;
+  for (int q = threadIdx.x; q < nv; q += blockDim.x) {
+    WideView w;
+    w.v = P.view_idx[vb + q];
+    col_coef(P, w.v, w.Au, w.Bu, w.Cu);
+    w.pad = 0;
+    s_view[q] = w;
+  }
+  if (threadIdx.x < 32) {
+    (&S.t1[0][0])[threadIdx.x] = 0.0;
+    (&S.t2[0][0])[threadIdx.x] = 0.0f;
+  }
+  __syncthreads();
+
+  if (wave < nth) {
+    // ---------------- theta wave ----------------
+    const int q0 = wave * vpw, nq = max(0, min(vpw, nv - q0));
+    const int npair = nq * EPV;
+    const int *lst = list + n0;
+    unsigned long long nent = 0;
+    int pix_next = lst[0];
+    for (int n = 0; n < cnt; n++) {
+      const int par = n & 1;
+      const int pix = pix_next;
+      pix_next = lst[min(n + 1, cnt - 1)];
+      const int yy = pix >> 16, xx = pix & 0xffff;
+      float e[PMAX][R], w[PMAX][R], a[PMAX];
+      int off[PMAX];
+      // issue every pass's loads, then evaluate the A values while they fly
+      double u[PMAX];
+      int loc[PMAX], hic[PMAX], vv[PMAX];
+#pragma unroll
+      for (int p = 0; p < PMAX; p++) {
+        // this lane's entry of pass p: view q0 + lq, window bin j, row group g
+        const int i = p * 64 + lane;
+        const bool on = i < npair;
+        const int lq = on ? i / EPV : 0, rem = i - lq * EPV;
+        const int j = on ? rem / ngr : 0, g = on ? rem - (rem / ngr) * ngr : 0;
+        const WideView &wv = s_view[on ? q0 + lq : 0];
+        vv[p] = wv.v;
+        u[p] = col_u(xx, yy, wv.Au, wv.Bu, wv.Cu);
+        const int w0 = on ? col_window(P, u[p], NBX, loc[p], hic[p]) : -1;
+        loc[p] = w0 >= 0 ? loc[p] : 1;
+        hic[p] = w0 >= 0 ? hic[p] : 0;  // empty: no bin passes
+        off[p] = (int)(sidx(P, z, wv.v, max(w0, 0) + j, g * R) - sidx(P, z, 0, 0, 0));
+        if (p * 64 < npair) {
+          ld_r<R>(P.e + sidx(P, z, 0, 0, 0) + off[p], e[p]);
+          ld_r<R>(P.w + sidx(P, z, 0, 0, 0) + off[p], w[p]);
+        } else {
+#pragma unroll
+          for (int r = 0; r < R; r++) e[p][r] = w[p][r] = 0.0f;
+        }
+        // the window's bin of this lane: j past w0 (kept in loc / hic terms)
+        a[p] = 0.0f;
+        if (p * 64 < npair && w0 >= 0)
+          a[p] = col_value(P, P.hr + (size_t)vv[p] * kHrStride, u[p], w0 + j, loc[p], hic[p]);
+      }
+      // theta sums (XT_VoxUpdate.c:61-92) with the generalized-Huber weights
+      // (XT_VoxUpdate.c:66-80)
+      double t1 = 0.0;
+      float t2 = 0.0f;
+#pragma unroll
+      for (int p = 0; p < PMAX; p++) {
+        if (p * 64 >= npair) break;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const float ee = e[p][r], ww = w[p][r];
+          const float cw = huber_sel(P, ee, ww)
+                               ? ww
+                               : P.Td * __builtin_amdgcn_sqrtf(ww) * __builtin_amdgcn_rcpf(fabsf(ee));
+          const float ac = a[p] * cw;
+          t1 = t1 + (double)(-ee * ac);
+          t2 = t2 + a[p] * ac;
+        }
+        nent += a[p] > 0.0f ? R : 0;
+      }
+      wave_total_df(t1, t2);
+      if (lane == 0) {
+        S.t1[par][wave] = t1;
+        S.t2[par][wave] = t2;
+      }
+      __syncthreads();
+      float V;
+      bool skip;
+      const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
+      const float dx = xn - V;
+      if (dx != 0.0f) {
+#pragma unroll
+        for (int p = 0; p < PMAX; p++) {
+          if (p * 64 >= npair) break;
+          if (a[p] != 0.0f) {
+            float *q = P.e + sidx(P, z, 0, 0, 0) + off[p];
+            if constexpr (R == 1) {
+              q[0] = fmaf(-a[p], dx, e[p][0]);
+            } else {
+              *reinterpret_cast<float2 *>(q) =
+                  make_float2(fmaf(-a[p], dx, e[p][0]), fmaf(-a[p], dx, e[p][1]));
+            }
+          }
+        }
+      }
+    }
+    const double nsum = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
+  } else {
+    // ---------------- prior wave ----------------
+    prior_wave<MAXTH>(P, S, list, n0, cnt, f, z, iter, seq, count_mag, nth, lane);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    P.cell_ent[cell] += ne;
+  }
+}
+
+// 11 theta waves keep the passes' registers unspilled
+template __global__ void k_icd_wpass<17, 2, 16, 11>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<17, 2, 24, 11>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<9, 2, 16, 11>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<5, 2, 16, 11>(KP, const int2 *, int, int, int, int, int);
+
 template __global__ void k_icd_mwg<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 1, 8>(KP, const int2 *, int, int, int, int, int);
