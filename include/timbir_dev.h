@@ -54,7 +54,8 @@ typedef struct {
                                      applies, 5 = the wide-column
                                      multi-wavefront kernels (columns of
                                      up to 17 bins, views in passes when
-                                     there are many) where they apply */
+                                     there are many) where they apply,
+                                     6 = the pass-based one alone */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront

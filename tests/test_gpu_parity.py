@@ -34,13 +34,14 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1, 2, 3, 4, 5],
-                ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal", "wide"])
+@pytest.fixture(params=[0, 1, 2, 3, 4, 5, 6],
+                ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal", "wide", "wpass"])
 def kernel(request):
     """Voxel-update kernel: 0 = the fastest that applies, 1 = the one-voxel
     kernel everywhere, 2 = the one-wavefront LDS kernel, 3 = the
     multi-wavefront LDS kernel, 4 = the multi-wavefront kernel with e in
-    global memory, 5 = the wide-column multi-wavefront kernel."""
+    global memory, 5 = the wide-column multi-wavefront kernels, 6 = the
+    pass-based one alone."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -55,7 +56,7 @@ def check_kernel(g, kernel):
         assert st["lds_launches"] + st["global_launches"] == st["voxel_launches"], st
     elif kernel in (2, 3):
         assert st["lds_launches"] == st["voxel_launches"], st
-    elif kernel in (4, 5):
+    elif kernel in (4, 5, 6):
         assert st["global_launches"] == st["voxel_launches"], st
     else:
         assert st["lds_launches"] == 0 and st["global_launches"] == 0, st
@@ -176,8 +177,8 @@ def test_cost_matches_oracle(quad):
 
 
 def test_single_voxel_update(kernel):
-    if kernel == 5:
-        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
+    if kernel in (5, 6):
+        pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"single voxel kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40)
     o, g = pair(scan)
@@ -256,8 +257,8 @@ def test_vsc_select_lists():
 def test_icd_iterations_track_oracle(quad, rings, kernel):
     """Same colour schedule on both sides: the trajectories agree to FP32
     (non-stiff prior, see test_phases_track_oracle)."""
-    if kernel == 5:
-        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
+    if kernel in (5, 6):
+        pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"icd trajectory quad={quad} rings={rings} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, quad_convex=quad, rings=0.01 if rings else 0.0,
                       zingers=1e-3 if not quad else 0.0, remove_rings=rings, c_s=1.0, c_t=1.0)
@@ -280,8 +281,8 @@ def test_icd_iterations_track_oracle(quad, rings, kernel):
 def test_converges_to_map(kernel):
     """GPU ICD reaches the scipy minimiser of the explicit MAP cost."""
     import mathref
-    if kernel == 5:
-        pytest.skip("the wide-column kernel needs at least 17 detector bins")
+    if kernel in (5, 6):
+        pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"converge to MAP kernel={kernel}")
     scan = scan_small(N=12, nz=4, n_views=16)
     o = pyorc.Problem(scan, opt=pyorc.options(), precision="f64")
@@ -304,8 +305,8 @@ def test_phases_track_oracle(kernel, c=1.0):
     chaotic: two CPU oracles that differ only in float vs double arrays part
     by ~1% RMSE at iteration 2.  Trajectory parity is therefore checked with
     c = 1; stiff problems are checked at convergence (test_converges_to_map)."""
-    if kernel == 5:
-        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
+    if kernel in (5, 6):
+        pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"phase by phase c={c} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, c_s=c, c_t=c)
     o, g = pair(scan)
@@ -373,8 +374,8 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
         pytest.skip("more views per cell than the global-e kernel's lanes hold")
     if nv > 256 and kernel in (2, 3):
         pytest.skip("the cell's (e, w) block does not fit in LDS")
-    if kernel == 5 and mz == 1:
-        pytest.skip("the wide-column kernel serves slices of 2 or more detector rows")
+    if kernel in (5, 6) and mz == 1:
+        pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
     n, nz = ((64, 16) if mxy == 8 else (64, 8) if mxy == 4 or n_views == 128
              else (32, 4))

@@ -33,7 +33,7 @@ template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int MAXTH>
 __global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
-template <int NBX, int R, int PMAX, int MAXTH>
+template <int NBX, int R, int PMAX, int MAXTH, bool HL>
 __global__ void k_icd_wpass(KP, const int2 *, int, int, int, int, int);
 template <int NB>
 __global__ void k_build_columns(KP);
@@ -992,8 +992,10 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     }
     if (fn && run_mw(fn, gnb, nth, vpw, 0, true) < 0) return -1;
   }
+  // kernel 6 asks for the pass form alone
+  const bool wpass_first = g_opt.kernel == 6;
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5) && 2 * st->mult_xy + 1 <= 17 &&
-      st->N_r >= 17) {
+      st->N_r >= 17 && !wpass_first) {
     // columns of up to 17 bins (the coarsest stages): a lane per (view, R
     // detector rows), the fewest rows per lane that fit in 8 theta waves
     const int mz = st->mult_z;
@@ -1012,20 +1014,30 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       launched = true;
     }
   }
-  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5) && st->mult_z % 2 == 0) {
-    // too many views for a lane per view: 11 theta waves own blocks of views
-    // and walk their (view, bin, 2 rows) entries in passes
+  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5 || g_opt.kernel == 6) &&
+      st->mult_z % 2 == 0) {
+    // 11 theta waves own blocks of views and walk their (view, bin, 2 rows)
+    // entries in passes; the views' detector responses go to LDS when they fit
     const int w = 2 * st->mult_xy + 1;
     const int nbx = w <= 5 ? 5 : w <= 9 ? 9 : w <= 17 ? 17 : 0;
     constexpr int nth = 11;
     const int vpw = (st->max_nv + nth - 1) / nth;
     const int passes = (vpw * nbx * (st->mult_z / 2) + 63) / 64;
-    const size_t lds = (size_t)st->max_nv * 32;  // per-view coefficients
+    // per-view coefficients, per-wave A values, then the detector responses
+    // when they fit
+    const size_t lds_c = (size_t)st->max_nv * 32 + (size_t)nth * vpw * (nbx + 1) * sizeof(float);
+    const size_t lds_h = lds_c + (size_t)st->max_nv * kHrStride * sizeof(float);
+    const bool hl = lds_h + 4096 <= st->lds_max;
+    const size_t lds = hl ? lds_h : lds_c;
     mw_fn fn = nullptr;
     if (nbx == 17)
-      fn = passes <= 16 ? k_icd_wpass<17, 2, 16, 11> : passes <= 24 ? k_icd_wpass<17, 2, 24, 11> : nullptr;
-    else if (nbx == 9 && passes <= 16) fn = k_icd_wpass<9, 2, 16, 11>;
-    else if (nbx == 5 && passes <= 16) fn = k_icd_wpass<5, 2, 16, 11>;
+      fn = passes <= 16 ? (hl ? k_icd_wpass<17, 2, 16, 11, true> : k_icd_wpass<17, 2, 16, 11, false>)
+         : passes <= 24 ? (hl ? k_icd_wpass<17, 2, 24, 11, true> : k_icd_wpass<17, 2, 24, 11, false>)
+                        : nullptr;
+    else if (nbx == 9 && passes <= 16)
+      fn = hl ? k_icd_wpass<9, 2, 16, 11, true> : k_icd_wpass<9, 2, 16, 11, false>;
+    else if (nbx == 5 && passes <= 16)
+      fn = hl ? k_icd_wpass<5, 2, 16, 11, true> : k_icd_wpass<5, 2, 16, 11, false>;
     if (fn && st->N_r >= nbx && lds + 4096 <= st->lds_max) {
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),

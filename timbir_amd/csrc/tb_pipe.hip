@@ -1390,12 +1390,13 @@ struct WideView {
   int pad;
 };
 
-template <int NBX, int R, int PMAX, int MAXTH>
+template <int NBX, int R, int PMAX, int MAXTH, bool HL>
 __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2 *__restrict__ cells,
                                                             int iter, int chunk, int seq,
                                                             int count_mag, int vpw) {
   __shared__ StepIn S;
-  extern __shared__ WideView s_view[];  // [views of the frame]
+  // [views of the frame] coefficients, then (HL) their detector responses
+  extern __shared__ WideView s_view[];
   const int2 cl = cells[blockIdx.x];
   const int f = cl.x, z = cl.y;
   const int cell = f * P.N_z + z;
@@ -1419,13 +1420,20 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
     cnt = 1;
   }
   if (cnt <= 0 || nv <= 0) return;
+  float *s_H = reinterpret_cast<float *>(s_view + nv);
+  // per theta wave: its views' A values at the NBX window bins, then the
+  // window start w0 (int bits), NBX + 1 floats per view
+  float *s_A = s_H + (HL ? nv * kHrStride : 0);
   for (int q = threadIdx.x; q < nv; q += blockDim.x) {
     WideView w;
     w.v = P.view_idx[vb + q];
     col_coef(P, w.v, w.Au, w.Bu, w.Cu);
-    w.pad = 0;
+    w.pad = q;
     s_view[q] = w;
   }
+  if constexpr (HL)
+    for (int i = threadIdx.x; i < nv * kHrStride; i += blockDim.x)
+      s_H[i] = P.hr[(size_t)P.view_idx[vb + i / kHrStride] * kHrStride + i % kHrStride];
   if (threadIdx.x < 32) {
     (&S.t1[0][0])[threadIdx.x] = 0.0;
     (&S.t2[0][0])[threadIdx.x] = 0.0f;
@@ -1446,9 +1454,20 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
       const int yy = pix >> 16, xx = pix & 0xffff;
       float e[PMAX][R], w[PMAX][R], a[PMAX];
       int off[PMAX];
-      // issue every pass's loads, then evaluate the A values while they fly
-      double u[PMAX];
-      int loc[PMAX], hic[PMAX], vv[PMAX];
+      // the wave's views: window and A values once per (view, bin), in LDS
+      float *sa = s_A + wave * vpw * (NBX + 1);
+      for (int i = lane; i < nq * NBX; i += 64) {
+        const int lq = i / NBX, j = i - lq * NBX;
+        const WideView &wv = s_view[q0 + lq];
+        const double u = col_u(xx, yy, wv.Au, wv.Bu, wv.Cu);
+        int loc, hic;
+        const int w0 = col_window(P, u, NBX, loc, hic);
+        const float *H = (HL ? s_H + (size_t)(q0 + lq) * kHrStride
+                             : P.hr + (size_t)wv.v * kHrStride);
+        sa[lq * (NBX + 1) + j] = w0 >= 0 ? col_value(P, H, u, w0 + j, loc, hic) : 0.0f;
+        if (j == 0) sa[lq * (NBX + 1) + NBX] = __int_as_float(w0);
+      }
+      // passes: this wave's LDS writes above complete before its reads
 #pragma unroll
       for (int p = 0; p < PMAX; p++) {
         // this lane's entry of pass p: view q0 + lq, window bin j, row group g
@@ -1456,13 +1475,10 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
         const bool on = i < npair;
         const int lq = on ? i / EPV : 0, rem = i - lq * EPV;
         const int j = on ? rem / ngr : 0, g = on ? rem - (rem / ngr) * ngr : 0;
-        const WideView &wv = s_view[on ? q0 + lq : 0];
-        vv[p] = wv.v;
-        u[p] = col_u(xx, yy, wv.Au, wv.Bu, wv.Cu);
-        const int w0 = on ? col_window(P, u[p], NBX, loc[p], hic[p]) : -1;
-        loc[p] = w0 >= 0 ? loc[p] : 1;
-        hic[p] = w0 >= 0 ? hic[p] : 0;  // empty: no bin passes
-        off[p] = (int)(sidx(P, z, wv.v, max(w0, 0) + j, g * R) - sidx(P, z, 0, 0, 0));
+        const int v = s_view[on ? q0 + lq : 0].v;
+        const int w0 = on ? __float_as_int(sa[lq * (NBX + 1) + NBX]) : -1;
+        a[p] = w0 >= 0 ? sa[lq * (NBX + 1) + j] : 0.0f;
+        off[p] = (int)(sidx(P, z, v, max(w0, 0) + j, g * R) - sidx(P, z, 0, 0, 0));
         if (p * 64 < npair) {
           ld_r<R>(P.e + sidx(P, z, 0, 0, 0) + off[p], e[p]);
           ld_r<R>(P.w + sidx(P, z, 0, 0, 0) + off[p], w[p]);
@@ -1470,29 +1486,28 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
 #pragma unroll
           for (int r = 0; r < R; r++) e[p][r] = w[p][r] = 0.0f;
         }
-        // the window's bin of this lane: j past w0 (kept in loc / hic terms)
-        a[p] = 0.0f;
-        if (p * 64 < npair && w0 >= 0)
-          a[p] = col_value(P, P.hr + (size_t)vv[p] * kHrStride, u[p], w0 + j, loc[p], hic[p]);
       }
       // theta sums (XT_VoxUpdate.c:61-92) with the generalized-Huber weights
       // (XT_VoxUpdate.c:66-80)
       double t1 = 0.0;
       float t2 = 0.0f;
+      // full unrolling keeps the pass arrays in registers: no early exit
 #pragma unroll
       for (int p = 0; p < PMAX; p++) {
-        if (p * 64 >= npair) break;
+        if (p * 64 < npair) {
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-          const float ee = e[p][r], ww = w[p][r];
-          const float cw = huber_sel(P, ee, ww)
-                               ? ww
-                               : P.Td * __builtin_amdgcn_sqrtf(ww) * __builtin_amdgcn_rcpf(fabsf(ee));
-          const float ac = a[p] * cw;
-          t1 = t1 + (double)(-ee * ac);
-          t2 = t2 + a[p] * ac;
+          for (int r = 0; r < R; r++) {
+            const float ee = e[p][r], ww = w[p][r];
+            const float cw = huber_sel(P, ee, ww)
+                                 ? ww
+                                 : P.Td * __builtin_amdgcn_sqrtf(ww) *
+                                       __builtin_amdgcn_rcpf(fabsf(ee));
+            const float ac = a[p] * cw;
+            t1 = t1 + (double)(-ee * ac);
+            t2 = t2 + a[p] * ac;
+          }
+          nent += a[p] > 0.0f ? R : 0;
         }
-        nent += a[p] > 0.0f ? R : 0;
       }
       wave_total_df(t1, t2);
       if (lane == 0) {
@@ -1507,8 +1522,7 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
       if (dx != 0.0f) {
 #pragma unroll
         for (int p = 0; p < PMAX; p++) {
-          if (p * 64 >= npair) break;
-          if (a[p] != 0.0f) {
+          if (p * 64 < npair && a[p] != 0.0f) {
             float *q = P.e + sidx(P, z, 0, 0, 0) + off[p];
             if constexpr (R == 1) {
               q[0] = fmaf(-a[p], dx, e[p][0]);
@@ -1535,10 +1549,14 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
 }
 
 // 11 theta waves keep the passes' registers unspilled
-template __global__ void k_icd_wpass<17, 2, 16, 11>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_wpass<17, 2, 24, 11>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_wpass<9, 2, 16, 11>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_wpass<5, 2, 16, 11>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<17, 2, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<17, 2, 24, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<9, 2, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<5, 2, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<17, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<17, 2, 24, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<9, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<5, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
 
 template __global__ void k_icd_mwg<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
