@@ -1,23 +1,29 @@
-// LDS-resident ICD voxel update for gfx950: one wavefront per cell, lane = view.
+// ICD voxel-update kernels for gfx950 (updateVoxels_Atten and
+// compute_voxel_update_Atten, XT_VoxUpdate.c:43-289), one workgroup per cell.
 //
-// Every error-sinogram entry a cell (frame f, slice z) touches lies in the
-// block [views of f][N_r][rows of z], and no other cell of the same colour
-// touches it.  When that block fits in LDS (64 views x 256 bins x (e, w) is
-// 128 KB of the 160 KB), the workgroup copies it in, runs the cell's whole
-// voxel sequence against LDS, and copies e back at the end.  LDS operations of
-// one wavefront complete in order, so voxel k+1 reads exactly the e that
-// voxel k wrote: the sequential sweep of updateVoxels_Atten
-// (XT_VoxUpdate.c:117-281) is reproduced without atomics or bookkeeping.
+// A cell is (frame f, slice z).  Every error-sinogram entry it touches lies
+// in the block [views of f][N_r][rows of z], and no other cell of the same
+// colour (f + z) & 1 touches it, so cells of one colour run side by side with
+// no atomics.  Inside a cell the voxels are updated one after another, and
+// each kernel keeps that sequence exact by letting only one wavefront read or
+// write a given e entry: the operations of one wavefront to one address, in
+// LDS or in global memory, take effect in issue order.
 //
-// What stays in global memory is the object: the 28 neighbour values each
-// voxel needs.  They are loaded one group of G voxels ahead.  An in-plane
-// neighbour may be a voxel updated after its load was issued; every update
-// patches the in-flight copies of the voxels that have it as a neighbour.
-// The loop is laid out so that its one wait for global memory,
-// `s_waitcnt vmcnt(0)`, comes a whole group after the loads it covers:
-//   fill group g (column footprints, neighbour loads, next group's pixels)
-//   solve group g-1 from registers and LDS
-//   wait; store group g-1's voxel values; group g becomes current.
+//   k_icd_lds    one wavefront per cell, lane = view, the (e, w) block in LDS
+//   k_icd_mw     theta waves (views split between them) + a prior wave, the
+//                (e, w) block in LDS, the A values from the column table
+//   k_icd_mwg    the same roles with e and w in global memory, for blocks
+//                that do not fit in LDS (coarse stages, large finest stages)
+//   k_icd_wide   columns of up to 17 bins (coarsest stages): lane = (view,
+//                detector rows), A values evaluated in the lane
+//   k_icd_wpass  many views per cell: waves own blocks of views and walk
+//                their entries in coalesced passes
+//
+// In the multi-wavefront kernels every wave publishes its partial theta sums
+// in LDS, all waves meet at one barrier per voxel, and each evaluates the
+// same step from the same sums, so all hold the identical update.  The prior
+// wave loads the 28 neighbour values one group of voxels ahead and patches
+// in-flight copies when a neighbour is updated in between.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "tb_common.h"
@@ -87,6 +93,16 @@ __device__ __forceinline__ bool huber_sel(const KP &P, float e, float w) {
 
 constexpr int kNoPix = 0x7fffffff;  // empty slot (never equals a packed pixel)
 
+// One wavefront per cell, lane = view (VPL views per lane).  The workgroup
+// copies the cell's (e, w) block into LDS (64 views x 256 bins x (e, w) is
+// 128 KB of the 160 KB), runs the cell's whole voxel sequence against it and
+// copies e back at the end.  The 28 neighbour values of each voxel stay in
+// global memory and are loaded one group of G voxels ahead.  The loop is laid
+// out so that its one wait for global memory, s_waitcnt vmcnt(0), comes a
+// whole group after the loads it covers:
+//   fill group g (column footprints, neighbour loads, next group's pixels)
+//   solve group g-1 from registers and LDS
+//   wait; store group g-1's voxel values; group g becomes current.
 template <int VPL, int NB, int MZ, int G>
 __global__ __launch_bounds__(64) void k_icd_lds(KP P, const int2 *__restrict__ cells, int iter,
                                                 int chunk, int seq, int count_mag) {
