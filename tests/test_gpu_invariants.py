@@ -1,0 +1,71 @@
+"""Size-independent checks of the voxel kernels at sizes the CPU oracle cannot
+run in a test.
+
+ICD keeps the error sinogram e = y - A x - d up to date voxel by voxel
+(XT_VoxUpdate.c:94-114): after any number of updates, the e the kernels
+maintain must equal y - A x - d recomputed from the current object with the
+forward projector (XT_ForwardProject.c:8-25) and the current offsets, up to
+FP32 rounding.  A lost update, a race between cells or a wrong entry shows
+as an O(1) error somewhere; the test reports the worst entry."""
+import sys
+
+import numpy as np
+import pytest
+
+import timbir_amd as tb
+from timbir_amd import datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def check_invariant(scan, mult_xy, mult_z, iters=2):
+    st = tb.Stage(scan, mult_xy=mult_xy, mult_z=mult_z, stage_idx=0)
+    i = st.info
+    st.init_error_sino()
+    st.reset_stats()
+    for it in range(1, iters + 1):
+        st.icd_iteration(it)
+    s = st.stats()
+    e = st.error_sino().astype(np.float64)            # [N_p][N_r][N_t]
+    ax = st.forward_project().astype(np.float64)
+    d = st.offset().astype(np.float64)                # [N_r][N_t]
+    y = np.asarray(scan["projections"], np.float64)   # [N_p][rows][cols]
+    y = np.transpose(y, (0, 2, 1))                    # [N_p][N_r][N_t]
+    ref = y - ax - d[None]
+    st.close()
+    err = np.abs(e - ref)
+    rel = float(np.sqrt(np.mean((e - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    log(f"  stage ({mult_xy},{mult_z}) N_z={i['N_z']} N_time={i['N_time']} views/frame="
+        f"{i['N_p'] // i['N_time']}: {s['updates']} updates, {s['lds_launches']} LDS / "
+        f"{s['global_launches']} global launches; e vs y - Ax - d: rms {rel:.2e} of rms, "
+        f"max {err.max():.2e} (|y| max {np.abs(y).max():.2f})")
+    return s, rel, float(err.max()), float(np.abs(y).max())
+
+
+def test_error_sinogram_stays_consistent_global_e():
+    """Finest stage whose cells (128 views x 256 bins) exceed LDS: the
+    global-e kernel, 256 x 256 x 16 slices x 8 frames, 1024 views."""
+    log("invariant: global-e kernel")
+    scan = datagen.interlaced4d(N=256, nz=16, n_frames=8, n_views=1024, ch_steps=16,
+                                frame_steps=4, seed=11)
+    s, rel, emax, ymax = check_invariant(scan, 1, 1)
+    assert s["global_launches"] == s["voxel_launches"] > 0
+    assert s["updates"] > 0
+    assert rel < 1e-4 and emax < 1e-3 * ymax
+
+
+def test_error_sinogram_stays_consistent_wide_columns():
+    """Coarsest stage of a 512-wide scan: 17-bin columns over 8 detector rows
+    (the wide-column kernel), 512 x 512 x 16 slices x 2 frames, 128 views."""
+    log("invariant: wide-column kernel")
+    scan = datagen.interlaced4d(N=512, nz=16, n_frames=2, n_views=128, ch_steps=8,
+                                frame_steps=2, seed=12)
+    s, rel, emax, ymax = check_invariant(scan, 8, 8)
+    assert s["global_launches"] == s["voxel_launches"] > 0
+    assert s["updates"] > 0
+    assert rel < 1e-4 and emax < 1e-3 * ymax
