@@ -69,3 +69,14 @@ def test_error_sinogram_stays_consistent_wide_columns():
     assert s["global_launches"] == s["voxel_launches"] > 0
     assert s["updates"] > 0
     assert rel < 1e-4 and emax < 1e-3 * ymax
+
+
+def test_error_sinogram_stays_consistent_many_views():
+    """Single-frame scan with 720 views at its (2,2) stage: more views than
+    any lane layout holds, so the pass-based kernel runs, 256 x 256 x 4."""
+    log("invariant: pass-based kernel")
+    scan = datagen.batched(n_problems=1, sizes=(256,), views=(720,), seed=13)[0]
+    s, rel, emax, ymax = check_invariant(scan, 2, 2)
+    assert s["global_launches"] == s["voxel_launches"] > 0
+    assert s["updates"] > 0
+    assert rel < 1e-4 and emax < 1e-3 * ymax
