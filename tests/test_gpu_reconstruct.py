@@ -232,3 +232,31 @@ def test_precision_scan_follows_oracle(tmp_path, monkeypatch, options):
     monkeypatch.chdir(tmp_path)
     log("precision3d 32x32x8, 48 views")
     check_workload(datagen.precision3d(N=32, nz=8, n_views=48, convg_thresh=0.0))
+
+
+def test_bin_data_driver_on_plain3d(tmp_path, monkeypatch, options):
+    """The plain 3D workload at its full size (128 x 128 x 4, 256 views)
+    through the bin_data files and the driver, against reconstruct() on the
+    same arrays: identical objects and offsets."""
+    log("bin_data driver, plain3d 128x128x4")
+    scan = datagen.plain3d()
+    for k in ("vox_wid", "rot_center", "sig_s", "sig_t", "c_s", "c_t", "convg_thresh",
+              "huber_delta", "huber_T"):
+        scan[k] = float(np.float32(scan[k]))
+    d1 = tmp_path / "driver"
+    d2 = tmp_path / "library"
+    d1.mkdir()
+    d2.mkdir()
+    bindata.write(scan, d1)
+    r = subprocess.run(bindata.command_line(scan), cwd=d1, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    monkeypatch.chdir(d2)
+    tb.set_options(verbose=0)
+    x = tb.reconstruct(scan)
+    out = bindata.read_outputs(d1, scan["proj_rows"], scan["proj_cols"], scan["recon_num"])
+    np.testing.assert_array_equal(out["object"], x)
+    np.testing.assert_array_equal(
+        out["proj_offset"],
+        bindata.read_outputs(d2, scan["proj_rows"], scan["proj_cols"], 1)["proj_offset"])
+    assert out["status"] == 1
