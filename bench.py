@@ -191,6 +191,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    ap.add_argument("--deploy4d", action="store_true",
+                    help="also time the multires schedule to the stop rule on the deployment-"
+                         "size scan (512x512x32 x 16 frames, 2048 views; about a minute to "
+                         "generate)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -267,6 +271,13 @@ def main():
         log("time to converged from the filtered back-projection ...")
         line["time_to_converged_fbp_init"] = time_to_converged(
             scan, finest_cap=args.finest_cap if args.converge else 0, init="fbp")
+    if args.deploy4d:
+        from timbir_amd import datagen
+        log("deployment-size scan ...")
+        big = datagen.deploy4d()
+        log("time to converged, deployment size ...")
+        line["time_to_converged_deploy4d"] = time_to_converged(big)
+        del big
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)
