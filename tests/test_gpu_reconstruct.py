@@ -260,3 +260,27 @@ def test_bin_data_driver_on_plain3d(tmp_path, monkeypatch, options):
         out["proj_offset"],
         bindata.read_outputs(d2, scan["proj_rows"], scan["proj_cols"], 1)["proj_offset"])
     assert out["status"] == 1
+
+
+def test_cost_file_holds_the_last_stage(tmp_path, monkeypatch, options):
+    """With the cost enabled (the reference built without -DNO_COST_CALCULATE)
+    cost.bin holds the last stage's costs only: the initial cost, then one per
+    iteration (XT_ICD_update.c:1168, :1205), equal to the stage pipeline's."""
+    monkeypatch.chdir(tmp_path)
+    log("cost.bin")
+    scan = scan4d()
+    tb.set_options(max_iterations=3, verbose=0, compute_cost=1)
+    tb.reconstruct(scan)
+    costs = np.fromfile("cost.bin", np.float64)
+    assert len(costs) == 4
+    tb.set_options(compute_cost=0)
+    g0 = tb.Stage(scan, mult_xy=2, mult_z=2, stage_idx=0)
+    g0.stage_run(3, with_cost=False)
+    g1 = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=2, var_est=g0.var_est)
+    g1.init_from_stage(g0)
+    g1.set_offset(g0.offset())
+    g0.close()
+    c1 = g1.stage_run(3, with_cost=True)
+    g1.close()
+    np.testing.assert_array_equal(costs, c1)
+    assert costs[-1] < costs[0]
