@@ -2,6 +2,8 @@
 import ctypes as C
 import os
 import re
+import subprocess
+import sys
 
 import pytest
 
@@ -72,7 +74,6 @@ DRIVER = os.path.join(ROOT, "timbir_amd", "tb_bin_data")
 def test_bin_data_driver_rejects_missing_input(tmp_path):
     """The bin_data driver reads its inputs before any GPU call: a missing
     projections.bin ends it with status 1 and the reference's message."""
-    import subprocess
     assert os.path.exists(DRIVER), "build() makes timbir_amd/tb_bin_data"
     r = subprocess.run([DRIVER, "--proj_rows", "4", "--proj_cols", "16", "--proj_num", "8",
                         "--recon_num", "1", "--vox_wid", "1", "--rot_center", "8"],
@@ -83,8 +84,15 @@ def test_bin_data_driver_rejects_missing_input(tmp_path):
 
 
 def test_bin_data_driver_rejects_bad_arguments(tmp_path):
-    import subprocess
     r = subprocess.run([DRIVER, "--proj_rows", "4", "extra"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1
     assert "ERROR: read_command_line_args" in r.stdout
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the bench exits with a message, before any device work, for N > 1
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "exactly one GPU" in r.stdout
