@@ -80,3 +80,14 @@ def test_error_sinogram_stays_consistent_many_views():
     assert s["global_launches"] == s["voxel_launches"] > 0
     assert s["updates"] > 0
     assert rel < 1e-4 and emax < 1e-3 * ymax
+
+
+def test_error_sinogram_stays_consistent_single_row_passes():
+    """Finest stage of the precision-study model with 1440 views in one frame
+    (more than any lane layout holds): single-row passes, 128 x 128 x 8."""
+    log("invariant: single-row pass-based kernel")
+    scan = datagen.precision3d(N=128, nz=8, n_views=1440, seed=14)
+    s, rel, emax, ymax = check_invariant(scan, 1, 1)
+    assert s["global_launches"] == s["voxel_launches"] > 0
+    assert s["updates"] > 0
+    assert rel < 1e-4 and emax < 1e-3 * ymax

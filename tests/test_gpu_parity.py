@@ -359,6 +359,7 @@ def test_phases_track_oracle(kernel, c=1.0):
     (4, 2, 400, 2),   # 200 views per frame: two views per lane
     (8, 8, 64, 2),    # 17 bins x 8 rows per view: one row per lane, 4 theta waves
     (8, 8, 256, 2),   # 128 views per frame: two rows per lane, 8 theta waves
+    (1, 1, 2000, 1),  # 2000 views in one frame: single-row passes (k_icd_wpass)
     (2, 2, 1200, 2),  # 600 views per frame: views walked in passes (k_icd_wpass)
     (8, 8, 320, 2),   # 160 views per frame x 8 rows: passes of 17 bins x 2 rows
 ])
@@ -370,6 +371,8 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
         pytest.skip("the column table holds at most 9 bins")
     nv = n_views // n_frames
     mwg_views = {1: 924, 2: 528, 4: 210}.get(mxy, 0)
+    if kernel == 1 and nv > 1000:
+        pytest.skip("the one-voxel kernel takes minutes on this many views")
     if kernel == 4 and nv > mwg_views:
         pytest.skip("more views per cell than the global-e kernel's lanes hold")
     if nv > 256 and kernel in (2, 3):

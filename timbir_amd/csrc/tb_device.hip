@@ -1015,14 +1015,16 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     }
   }
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5 || g_opt.kernel == 6) &&
-      st->mult_z % 2 == 0) {
-    // 11 theta waves own blocks of views and walk their (view, bin, 2 rows)
-    // entries in passes; the views' detector responses go to LDS when they fit
+      (st->mult_z % 2 == 0 || st->mult_z == 1)) {
+    // 11 theta waves own blocks of views and walk their (view, bin, R rows)
+    // entries in passes (R = 2, or 1 for single-row slices); the views'
+    // detector responses go to LDS when they fit
+    const int R = st->mult_z % 2 == 0 ? 2 : 1;
     const int w = 2 * st->mult_xy + 1;
-    const int nbx = w <= 5 ? 5 : w <= 9 ? 9 : w <= 17 ? 17 : 0;
+    const int nbx = R == 1 ? (w <= 3 ? 3 : 0) : w <= 5 ? 5 : w <= 9 ? 9 : w <= 17 ? 17 : 0;
     constexpr int nth = 11;
     const int vpw = (st->max_nv + nth - 1) / nth;
-    const int passes = (vpw * nbx * (st->mult_z / 2) + 63) / 64;
+    const int passes = (vpw * nbx * (st->mult_z / R) + 63) / 64;
     // per-view coefficients, per-wave A values, then the detector responses
     // when they fit
     const size_t lds_c = (size_t)st->max_nv * 32 + (size_t)nth * vpw * (nbx + 1) * sizeof(float);
@@ -1038,6 +1040,8 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       fn = hl ? k_icd_wpass<9, 2, 16, 11, true> : k_icd_wpass<9, 2, 16, 11, false>;
     else if (nbx == 5 && passes <= 16)
       fn = hl ? k_icd_wpass<5, 2, 16, 11, true> : k_icd_wpass<5, 2, 16, 11, false>;
+    else if (nbx == 3 && passes <= 16)
+      fn = hl ? k_icd_wpass<3, 1, 16, 11, true> : k_icd_wpass<3, 1, 16, 11, false>;
     if (fn && st->N_r >= nbx && lds + 4096 <= st->lds_max) {
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),

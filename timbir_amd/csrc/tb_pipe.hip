@@ -1573,6 +1573,9 @@ template __global__ void k_icd_wpass<17, 2, 16, 11, true>(KP, const int2 *, int,
 template __global__ void k_icd_wpass<17, 2, 24, 11, true>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_wpass<9, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_wpass<5, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
+// single-row slices (the finest stage) with more views than k_icd_mwg holds
+template __global__ void k_icd_wpass<3, 1, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass<3, 1, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
 
 template __global__ void k_icd_mwg<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
