@@ -44,8 +44,9 @@ typedef struct {
                                      that applies (LDS-resident multi-
                                      wavefront, then multi-wavefront with e
                                      in global memory, then wide-column
-                                     multi-wavefront, then one-wavefront,
-                                     then one-voxel), 1 = the one-voxel
+                                     multi-wavefront, then pass-based,
+                                     then one-wavefront, then one-voxel),
+                                     1 = the one-voxel
                                      kernel only, 2 = the one-wavefront LDS
                                      kernel where it fits, 3 = the
                                      multi-wavefront LDS kernel where it
@@ -55,7 +56,9 @@ typedef struct {
                                      multi-wavefront kernels (columns of
                                      up to 17 bins, views in passes when
                                      there are many) where they apply,
-                                     6 = the pass-based one alone */
+                                     6 = the pass-based ones alone, 7 =
+                                     the pass-based one that walks any
+                                     number of passes in groups */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront

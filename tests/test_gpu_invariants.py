@@ -91,3 +91,16 @@ def test_error_sinogram_stays_consistent_single_row_passes():
     assert s["global_launches"] == s["voxel_launches"] > 0
     assert s["updates"] > 0
     assert rel < 1e-4 and emax < 1e-3 * ymax
+
+
+@pytest.mark.parametrize("mxy,mz", [(4, 4), (8, 8)])
+def test_error_sinogram_stays_consistent_grouped_passes(mxy, mz):
+    """Coarse stages of the precision-study model with 1440 views in one
+    frame: more passes than registers hold, so the grouped-pass kernel runs,
+    256 x 256 x 16."""
+    log(f"invariant: grouped-pass kernel ({mxy},{mz})")
+    scan = datagen.precision3d(N=256, nz=16, n_views=1440, seed=15)
+    s, rel, emax, ymax = check_invariant(scan, mxy, mz)
+    assert s["global_launches"] == s["voxel_launches"] > 0
+    assert s["updates"] > 0
+    assert rel < 1e-4 and emax < 1e-3 * ymax

@@ -34,14 +34,15 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1, 2, 3, 4, 5, 6],
-                ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal", "wide", "wpass"])
+@pytest.fixture(params=[0, 1, 2, 3, 4, 5, 6, 7],
+                ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal", "wide", "wpass",
+                     "wloop"])
 def kernel(request):
     """Voxel-update kernel: 0 = the fastest that applies, 1 = the one-voxel
     kernel everywhere, 2 = the one-wavefront LDS kernel, 3 = the
     multi-wavefront LDS kernel, 4 = the multi-wavefront kernel with e in
     global memory, 5 = the wide-column multi-wavefront kernels, 6 = the
-    pass-based one alone."""
+    pass-based ones alone, 7 = the looped pass kernel alone."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -56,7 +57,7 @@ def check_kernel(g, kernel):
         assert st["lds_launches"] + st["global_launches"] == st["voxel_launches"], st
     elif kernel in (2, 3):
         assert st["lds_launches"] == st["voxel_launches"], st
-    elif kernel in (4, 5, 6):
+    elif kernel in (4, 5, 6, 7):
         assert st["global_launches"] == st["voxel_launches"], st
     else:
         assert st["lds_launches"] == 0 and st["global_launches"] == 0, st
@@ -362,6 +363,8 @@ def test_phases_track_oracle(kernel, c=1.0):
     (1, 1, 2000, 1),  # 2000 views in one frame: single-row passes (k_icd_wpass)
     (2, 2, 1200, 2),  # 600 views per frame: views walked in passes (k_icd_wpass)
     (8, 8, 320, 2),   # 160 views per frame x 8 rows: passes of 17 bins x 2 rows
+    (4, 4, 1400, 2),  # 700 views per frame x 4 rows: more passes than registers (k_icd_wloop)
+    (8, 8, 600, 2),   # 300 views per frame x 8 rows: groups of passes (k_icd_wloop)
 ])
 def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
     """Every instantiation of the voxel-update kernels follows the oracle."""

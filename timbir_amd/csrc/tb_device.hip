@@ -35,6 +35,8 @@ template <int NBX, int R, int MAXTH>
 __global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int PMAX, int MAXTH, bool HL>
 __global__ void k_icd_wpass(KP, const int2 *, int, int, int, int, int);
+template <int NBX, int R, int GP, int MAXTH, bool HL>
+__global__ void k_icd_wloop(KP, const int2 *, int, int, int, int, int);
 template <int NB>
 __global__ void k_build_columns(KP);
 __global__ void k_offset_partial(KP, int, int, double *, double *);
@@ -1052,7 +1054,32 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       launched = true;
     }
   }
-
+  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5 || g_opt.kernel == 6 ||
+                    g_opt.kernel == 7) &&
+      (st->mult_z % 2 == 0 || st->mult_z == 1)) {
+    // more passes than k_icd_wpass holds (the coarse stages of scans with
+    // many views per frame): 15 theta waves walk their entries in groups
+    // of 8 passes and read e again for the write
+    const int R = st->mult_z % 2 == 0 ? 2 : 1;
+    const int w = 2 * st->mult_xy + 1;
+    const int nbx = R == 1 ? (w <= 3 ? 3 : 0) : w <= 5 ? 5 : w <= 9 ? 9 : w <= 17 ? 17 : 0;
+    const int nth = std::min(15, std::max(1, st->max_nv));
+    const int vpw = (st->max_nv + nth - 1) / nth;
+    const size_t lds = (size_t)st->max_nv * 32 + (size_t)nth * vpw * (nbx + 1) * sizeof(float);
+    mw_fn fn = nbx == 17 ? k_icd_wloop<17, 2, 8, 15, false>
+             : nbx == 9  ? k_icd_wloop<9, 2, 8, 15, false>
+             : nbx == 5  ? k_icd_wloop<5, 2, 8, 15, false>
+             : nbx == 3  ? k_icd_wloop<3, 1, 8, 15, false> : nullptr;
+    if (fn && st->N_r >= nbx && lds + 4096 <= st->lds_max) {
+      if (seq == 0 && ensure_order(st, iter)) return -1;
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
+                         chunk, seq, count_mag, vpw);
+      st->global_launches++;
+      launched = true;
+    }
+  }
 
   if (!launched && nb && g_opt.kernel != 1) {
     using lds_fn = void (*)(KP, const int2 *, int, int, int, int);

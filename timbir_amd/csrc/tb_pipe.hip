@@ -18,6 +18,8 @@
 //                detector rows), A values evaluated in the lane
 //   k_icd_wpass  many views per cell: waves own blocks of views and walk
 //                their entries in coalesced passes
+//   k_icd_wloop  k_icd_wpass with any number of passes, in groups, e read
+//                again for the write
 //
 // In the multi-wavefront kernels every wave publishes its partial theta sums
 // in LDS, all waves meet at one barrier per voxel, and each evaluates the
@@ -1563,6 +1565,188 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
     P.cell_ent[cell] += ne;
   }
 }
+
+// ---------------------------------------------------------------------------
+// k_icd_wloop: k_icd_wpass for any number of passes.  The wave walks its
+// entries in groups of GP passes (one group's e, w and A values in
+// registers at a time) and, after the step, reads e again to write
+// e - a dx.  No other wave writes the wave's entries between the two reads:
+// the columns of one voxel in one view are disjoint across views and each
+// (view, bin, row group) entry has one lane.  Serves the coarse stages of
+// scans with many views per frame (more passes than k_icd_wpass keeps in
+// registers); the detector responses stay in global memory.
+// ---------------------------------------------------------------------------
+template <int NBX, int R, int GP, int MAXTH, bool HL>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wloop(KP P, const int2 *__restrict__ cells,
+                                                            int iter, int chunk, int seq,
+                                                            int count_mag, int vpw) {
+  __shared__ StepIn S;
+  // [views of the frame] coefficients, then (HL) their detector responses
+  extern __shared__ WideView s_view[];
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+  const int ngr = P.mz / R;      // row groups of a view
+  const int EPV = NBX * ngr;     // lane entries of a view
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0 || nv <= 0) return;
+  float *s_H = reinterpret_cast<float *>(s_view + nv);
+  // per theta wave: its views' A values at the NBX window bins, then the
+  // window start w0 (int bits), NBX + 1 floats per view
+  float *s_A = s_H + (HL ? nv * kHrStride : 0);
+  for (int q = threadIdx.x; q < nv; q += blockDim.x) {
+    WideView w;
+    w.v = P.view_idx[vb + q];
+    col_coef(P, w.v, w.Au, w.Bu, w.Cu);
+    w.pad = q;
+    s_view[q] = w;
+  }
+  if constexpr (HL)
+    for (int i = threadIdx.x; i < nv * kHrStride; i += blockDim.x)
+      s_H[i] = P.hr[(size_t)P.view_idx[vb + i / kHrStride] * kHrStride + i % kHrStride];
+  if (threadIdx.x < 32) {
+    (&S.t1[0][0])[threadIdx.x] = 0.0;
+    (&S.t2[0][0])[threadIdx.x] = 0.0f;
+  }
+  __syncthreads();
+
+  if (wave < nth) {
+    // ---------------- theta wave ----------------
+    const int q0 = wave * vpw, nq = max(0, min(vpw, nv - q0));
+    const int npair = nq * EPV;
+    const int *lst = list + n0;
+    unsigned long long nent = 0;
+    int pix_next = lst[0];
+    for (int n = 0; n < cnt; n++) {
+      const int par = n & 1;
+      const int pix = pix_next;
+      pix_next = lst[min(n + 1, cnt - 1)];
+      const int yy = pix >> 16, xx = pix & 0xffff;
+      // the wave's views: window and A values once per (view, bin), in LDS
+      float *sa = s_A + wave * vpw * (NBX + 1);
+      for (int i = lane; i < nq * NBX; i += 64) {
+        const int lq = i / NBX, j = i - lq * NBX;
+        const WideView &wv = s_view[q0 + lq];
+        const double u = col_u(xx, yy, wv.Au, wv.Bu, wv.Cu);
+        int loc, hic;
+        const int w0 = col_window(P, u, NBX, loc, hic);
+        const float *H = (HL ? s_H + (size_t)(q0 + lq) * kHrStride
+                             : P.hr + (size_t)wv.v * kHrStride);
+        sa[lq * (NBX + 1) + j] = w0 >= 0 ? col_value(P, H, u, w0 + j, loc, hic) : 0.0f;
+        if (j == 0) sa[lq * (NBX + 1) + NBX] = __int_as_float(w0);
+      }
+      // entry i of the wave: view q0 + lq, window bin j, row group g
+      auto entry = [&](int i, float &a, int &off) {
+        const bool on = i < npair;
+        const int lq = on ? i / EPV : 0, rem = i - lq * EPV;
+        const int j = on ? rem / ngr : 0, g = on ? rem - (rem / ngr) * ngr : 0;
+        const int v = s_view[on ? q0 + lq : 0].v;
+        const int w0 = on ? __float_as_int(sa[lq * (NBX + 1) + NBX]) : -1;
+        a = w0 >= 0 ? sa[lq * (NBX + 1) + j] : 0.0f;
+        off = (int)(sidx(P, z, v, max(w0, 0) + j, g * R) - sidx(P, z, 0, 0, 0));
+      };
+      const float *eb = P.e + sidx(P, z, 0, 0, 0);
+      const float *wb = P.w + sidx(P, z, 0, 0, 0);
+      // theta sums (XT_VoxUpdate.c:61-92) with the generalized-Huber weights
+      // (XT_VoxUpdate.c:66-80), GP passes of 64 entries at a time
+      double t1 = 0.0;
+      float t2 = 0.0f;
+      for (int base = 0; base < npair; base += 64 * GP) {
+        float e[GP][R], w[GP][R], a[GP];
+        int off[GP];
+#pragma unroll
+        for (int p = 0; p < GP; p++) {
+          entry(base + p * 64 + lane, a[p], off[p]);
+          ld_r<R>(eb + off[p], e[p]);
+          ld_r<R>(wb + off[p], w[p]);
+        }
+#pragma unroll
+        for (int p = 0; p < GP; p++) {
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const float ee = e[p][r], ww = w[p][r];
+            const float cw = huber_sel(P, ee, ww)
+                                 ? ww
+                                 : P.Td * __builtin_amdgcn_sqrtf(ww) *
+                                       __builtin_amdgcn_rcpf(fabsf(ee));
+            const float ac = a[p] * cw;
+            t1 = t1 + (double)(-ee * ac);
+            t2 = t2 + a[p] * ac;
+          }
+          nent += a[p] > 0.0f ? R : 0;
+        }
+      }
+      wave_total_df(t1, t2);
+      if (lane == 0) {
+        S.t1[par][wave] = t1;
+        S.t2[par][wave] = t2;
+      }
+      __syncthreads();
+      float V;
+      bool skip;
+      const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
+      const float dx = xn - V;
+      if (dx != 0.0f) {
+        // this wave read those entries above and no other wave writes them:
+        // read them again and write e - a dx
+        for (int base = 0; base < npair; base += 64 * GP) {
+          float e[GP][R], a[GP];
+          int off[GP];
+#pragma unroll
+          for (int p = 0; p < GP; p++) {
+            entry(base + p * 64 + lane, a[p], off[p]);
+            ld_r<R>(eb + off[p], e[p]);
+          }
+#pragma unroll
+          for (int p = 0; p < GP; p++) {
+            if (a[p] != 0.0f) {
+              float *q = P.e + sidx(P, z, 0, 0, 0) + off[p];
+              if constexpr (R == 1) {
+                q[0] = fmaf(-a[p], dx, e[p][0]);
+              } else {
+                *reinterpret_cast<float2 *>(q) =
+                    make_float2(fmaf(-a[p], dx, e[p][0]), fmaf(-a[p], dx, e[p][1]));
+              }
+            }
+          }
+        }
+      }
+    }
+    const double nsum = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
+  } else {
+    // ---------------- prior wave ----------------
+    prior_wave<MAXTH>(P, S, list, n0, cnt, f, z, iter, seq, count_mag, nth, lane);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    P.cell_ent[cell] += ne;
+  }
+}
+
+// 15 theta waves: a group of 8 passes fits the registers of 4 waves per SIMD
+template __global__ void k_icd_wloop<17, 2, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wloop<9, 2, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wloop<5, 2, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wloop<3, 1, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
 
 // 11 theta waves keep the passes' registers unspilled
 template __global__ void k_icd_wpass<17, 2, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
