@@ -195,6 +195,9 @@ def main():
                     help="also time the multires schedule to the stop rule on the deployment-"
                          "size scan (512x512x32 x 16 frames, 2048 views; about a minute to "
                          "generate)")
+    ap.add_argument("--precision3d", action="store_true",
+                    help="also time the multires schedule to the stop rule on the single-"
+                         "frame precision-study scan (512x512x32, 1440 views)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -278,6 +281,10 @@ def main():
         log("time to converged, deployment size ...")
         line["time_to_converged_deploy4d"] = time_to_converged(big)
         del big
+    if args.precision3d:
+        from timbir_amd import datagen
+        log("time to converged, precision-study scan ...")
+        line["time_to_converged_precision3d"] = time_to_converged(datagen.precision3d())
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)
