@@ -104,3 +104,37 @@ def test_error_sinogram_stays_consistent_grouped_passes(mxy, mz):
     assert s["global_launches"] == s["voxel_launches"] > 0
     assert s["updates"] > 0
     assert rel < 1e-4 and emax < 1e-3 * ymax
+
+
+@pytest.mark.parametrize("mxy,mz", [(1, 1), (4, 4)])
+def test_forward_projection_conserves_mass(mxy, mz):
+    """Sum over the detector bins of every view of A x equals the object's
+    mass in that slice times delta_xy^2 / delta_r, whatever the view angle
+    (each A column, XT_AMatrix.c:50-110, spreads one voxel's area over the
+    bins; the 3-bin cap of the finest columns and the tabulated detector
+    response keep it within 1%).  Checks the device forward projector
+    (XT_ForwardProject.c:8-25) at a size and view count the oracle does not
+    run: 256 x 256 x 16 slices, 1440 views, object inside a disc so that no
+    footprint leaves the detector."""
+    log(f"forward projector mass ({mxy},{mz})")
+    scan = datagen.precision3d(N=256, nz=16, n_views=1440, seed=16)
+    st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=0)
+    i = st.info
+    n, nz = i["N_x"], i["N_z"]
+    c = (np.arange(n) + 0.5) - n / 2
+    disc = (c[:, None] ** 2 + c[None, :] ** 2) < (0.8 * n / 2) ** 2
+    x = (np.random.default_rng(17).random((i["N_time"], nz, n, n)) * disc).astype(np.float32)
+    st.set_object(x)
+    ax = st.forward_project().astype(np.float64)      # [N_p][N_r][N_t]
+    st.close()
+    per_row = ax.sum(axis=1)                          # [N_p][N_t]
+    rows_per_slice = i["N_t"] // nz
+    mass = x[0].astype(np.float64).sum(axis=(1, 2)) * i["delta_xy"] ** 2 / i["delta_r"]
+    # every detector row of slice z sees the slice's mass
+    want = np.repeat(mass, rows_per_slice)[None, :]
+    ratio = per_row / want
+    log(f"  {i['N_p']} views x {i['N_t']} rows: sum over bins / mass in "
+        f"[{ratio.min():.5f}, {ratio.max():.5f}]")
+    assert np.all(np.abs(ratio - 1.0) < 1e-2)
+    # the spread over views is the A columns' angular dependence only
+    assert ratio.max() - ratio.min() < 1.5e-2
