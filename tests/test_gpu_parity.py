@@ -178,7 +178,7 @@ def test_cost_matches_oracle(quad):
 
 
 def test_single_voxel_update(kernel):
-    if kernel in (5, 6):
+    if kernel == 5:
         pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"single voxel kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40)
@@ -258,7 +258,7 @@ def test_vsc_select_lists():
 def test_icd_iterations_track_oracle(quad, rings, kernel):
     """Same colour schedule on both sides: the trajectories agree to FP32
     (non-stiff prior, see test_phases_track_oracle)."""
-    if kernel in (5, 6):
+    if kernel == 5:
         pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"icd trajectory quad={quad} rings={rings} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, quad_convex=quad, rings=0.01 if rings else 0.0,
@@ -282,7 +282,7 @@ def test_icd_iterations_track_oracle(quad, rings, kernel):
 def test_converges_to_map(kernel):
     """GPU ICD reaches the scipy minimiser of the explicit MAP cost."""
     import mathref
-    if kernel in (5, 6):
+    if kernel == 5:
         pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"converge to MAP kernel={kernel}")
     scan = scan_small(N=12, nz=4, n_views=16)
@@ -306,7 +306,7 @@ def test_phases_track_oracle(kernel, c=1.0):
     chaotic: two CPU oracles that differ only in float vs double arrays part
     by ~1% RMSE at iteration 2.  Trajectory parity is therefore checked with
     c = 1; stiff problems are checked at convergence (test_converges_to_map)."""
-    if kernel in (5, 6):
+    if kernel == 5:
         pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"phase by phase c={c} kernel={kernel}")
     scan = scan_small(N=32, nz=4, n_views=40, c_s=c, c_t=c)
@@ -380,7 +380,7 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
         pytest.skip("more views per cell than the global-e kernel's lanes hold")
     if nv > 256 and kernel in (2, 3):
         pytest.skip("the cell's (e, w) block does not fit in LDS")
-    if kernel in (5, 6) and mz == 1:
+    if kernel == 5 and mz == 1:
         pytest.skip("the wide-column kernels serve slices of 2 or more detector rows")
     log(f"variant mxy={mxy} mz={mz} views={n_views} frames={n_frames} kernel={kernel}")
     n, nz = ((64, 16) if mxy == 8 else (64, 8) if mxy == 4 or n_views == 128

@@ -1045,6 +1045,12 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     else if (nbx == 3 && passes <= 16)
       fn = hl ? k_icd_wpass<3, 1, 16, 11, true> : k_icd_wpass<3, 1, 16, 11, false>;
     if (fn && st->N_r >= nbx && lds + 4096 <= st->lds_max) {
+      // single-row slices read their A values from the 3-bin column table
+      // when it fits in memory (the kernel computes them otherwise)
+      if (nbx == 3) {
+        if (ensure_columns(st, 3) < 0) return -1;
+        P = st->kp();
+      }
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

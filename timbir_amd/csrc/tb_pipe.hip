@@ -1474,6 +1474,19 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
       int off[PMAX];
       // the wave's views: window and A values once per (view, bin), in LDS
       float *sa = s_A + wave * vpw * (NBX + 1);
+      if (NBX == 3 && P.colt && P.colt_rec == NBX + 1) {
+        // the stage's 3-bin column table holds exactly these records, [A at
+        // w0 .. w0 + 2, w0], with the same arithmetic: copy them
+        const float4 *rec = reinterpret_cast<const float4 *>(P.colt) +
+                            ((size_t)f * P.N_x * P.N_y + (size_t)yy * P.N_x + xx) * P.colt_nv + q0;
+        for (int lq = lane; lq < nq; lq += 64) {
+          const float4 r4 = rec[lq];
+          sa[lq * 4 + 0] = r4.x;
+          sa[lq * 4 + 1] = r4.y;
+          sa[lq * 4 + 2] = r4.z;
+          sa[lq * 4 + 3] = r4.w;
+        }
+      } else
       for (int i = lane; i < nq * NBX; i += 64) {
         const int lq = i / NBX, j = i - lq * NBX;
         const WideView &wv = s_view[q0 + lq];
