@@ -879,7 +879,9 @@ static int ensure_columns(tb_stage *st, int nb) {
   KP P = st->kp();
   const size_t recs = n / rec;
   const int grid = (int)std::min<size_t>((recs + 255) / 256, 65536);
-  if (nb == 9) hipLaunchKernelGGL(k_build_columns<9>, dim3(grid), dim3(256), 0, st->stream, P);
+  if (nb == 17)
+    hipLaunchKernelGGL(k_build_columns<17>, dim3(grid), dim3(256), 0, st->stream, P);
+  else if (nb == 9) hipLaunchKernelGGL(k_build_columns<9>, dim3(grid), dim3(256), 0, st->stream, P);
   else if (nb == 5)
     hipLaunchKernelGGL(k_build_columns<5>, dim3(grid), dim3(256), 0, st->stream, P);
   else hipLaunchKernelGGL(k_build_columns<3>, dim3(grid), dim3(256), 0, st->stream, P);
@@ -1045,12 +1047,10 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     else if (nbx == 3 && passes <= 16)
       fn = hl ? k_icd_wpass<3, 1, 16, 11, true> : k_icd_wpass<3, 1, 16, 11, false>;
     if (fn && st->N_r >= nbx && lds + 4096 <= st->lds_max) {
-      // single-row slices read their A values from the 3-bin column table
-      // when it fits in memory (the kernel computes them otherwise)
-      if (nbx == 3) {
-        if (ensure_columns(st, 3) < 0) return -1;
-        P = st->kp();
-      }
+      // A values from the stage's nbx-bin column table when it fits in
+      // memory (the kernel evaluates them otherwise)
+      if (ensure_columns(st, nbx) < 0) return -1;
+      P = st->kp();
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1077,6 +1077,10 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
              : nbx == 5  ? k_icd_wloop<5, 2, 8, 15, false>
              : nbx == 3  ? k_icd_wloop<3, 1, 8, 15, false> : nullptr;
     if (fn && st->N_r >= nbx && lds + 4096 <= st->lds_max) {
+      // A values from the stage's nbx-bin column table when it fits in
+      // memory (the kernel evaluates them otherwise)
+      if (ensure_columns(st, nbx) < 0) return -1;
+      P = st->kp();
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

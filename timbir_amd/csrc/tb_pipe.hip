@@ -536,6 +536,7 @@ __global__ __launch_bounds__(256) void k_build_columns(KP P) {
 template __global__ void k_build_columns<3>(KP);
 template __global__ void k_build_columns<5>(KP);
 template __global__ void k_build_columns<9>(KP);
+template __global__ void k_build_columns<17>(KP);
 
 // The prior wave of the multi-wavefront kernels: the object side of the
 // voxel sequence (neighbour prefetch one group ahead with in-flight patching,
@@ -1474,17 +1475,39 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
       int off[PMAX];
       // the wave's views: window and A values once per (view, bin), in LDS
       float *sa = s_A + wave * vpw * (NBX + 1);
-      if (NBX == 3 && P.colt && P.colt_rec == NBX + 1) {
-        // the stage's 3-bin column table holds exactly these records, [A at
-        // w0 .. w0 + 2, w0], with the same arithmetic: copy them
-        const float4 *rec = reinterpret_cast<const float4 *>(P.colt) +
-                            ((size_t)f * P.N_x * P.N_y + (size_t)yy * P.N_x + xx) * P.colt_nv + q0;
-        for (int lq = lane; lq < nq; lq += 64) {
-          const float4 r4 = rec[lq];
-          sa[lq * 4 + 0] = r4.x;
-          sa[lq * 4 + 1] = r4.y;
-          sa[lq * 4 + 2] = r4.z;
-          sa[lq * 4 + 3] = r4.w;
+      if (P.colt && P.colt_rec == col_rec(NBX)) {
+        // the stage's NBX-bin column table holds these values, [A at w0 ..
+        // w0 + NBX - 1, w0] per view slot, with the same arithmetic: copy
+        constexpr int REC = col_rec(NBX);
+        const float *rec = P.colt + (((size_t)f * P.N_x * P.N_y + (size_t)yy * P.N_x + xx) *
+                                         P.colt_nv + q0) * REC;
+        if constexpr (NBX + 1 == REC) {
+          // whole records: one 16-byte load per view
+          const float4 *r4 = reinterpret_cast<const float4 *>(rec);
+          for (int lq = lane; lq < nq; lq += 64) {
+            const float4 r = r4[lq];
+            sa[lq * 4 + 0] = r.x;
+            sa[lq * 4 + 1] = r.y;
+            sa[lq * 4 + 2] = r.z;
+            sa[lq * 4 + 3] = r.w;
+          }
+        } else {
+          // 8 loads per lane in flight before their LDS stores
+          const int n = nq * (NBX + 1);
+          for (int i0 = 0; i0 < n; i0 += 64 * 8) {
+            float t[8];
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+              const int i = min(i0 + b * 64 + lane, n - 1);
+              const int lq = i / (NBX + 1);
+              t[b] = rec[lq * REC + (i - lq * (NBX + 1))];
+            }
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+              const int i = i0 + b * 64 + lane;
+              if (i < n) sa[i] = t[b];
+            }
+          }
         }
       } else
       for (int i = lane; i < nq * NBX; i += 64) {
@@ -1653,6 +1676,41 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wloop(KP P, const int2
       const int yy = pix >> 16, xx = pix & 0xffff;
       // the wave's views: window and A values once per (view, bin), in LDS
       float *sa = s_A + wave * vpw * (NBX + 1);
+      if (P.colt && P.colt_rec == col_rec(NBX)) {
+        // the stage's NBX-bin column table holds these values, [A at w0 ..
+        // w0 + NBX - 1, w0] per view slot, with the same arithmetic: copy
+        constexpr int REC = col_rec(NBX);
+        const float *rec = P.colt + (((size_t)f * P.N_x * P.N_y + (size_t)yy * P.N_x + xx) *
+                                         P.colt_nv + q0) * REC;
+        if constexpr (NBX + 1 == REC) {
+          // whole records: one 16-byte load per view
+          const float4 *r4 = reinterpret_cast<const float4 *>(rec);
+          for (int lq = lane; lq < nq; lq += 64) {
+            const float4 r = r4[lq];
+            sa[lq * 4 + 0] = r.x;
+            sa[lq * 4 + 1] = r.y;
+            sa[lq * 4 + 2] = r.z;
+            sa[lq * 4 + 3] = r.w;
+          }
+        } else {
+          // 8 loads per lane in flight before their LDS stores
+          const int n = nq * (NBX + 1);
+          for (int i0 = 0; i0 < n; i0 += 64 * 8) {
+            float t[8];
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+              const int i = min(i0 + b * 64 + lane, n - 1);
+              const int lq = i / (NBX + 1);
+              t[b] = rec[lq * REC + (i - lq * (NBX + 1))];
+            }
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+              const int i = i0 + b * 64 + lane;
+              if (i < n) sa[i] = t[b];
+            }
+          }
+        }
+      } else
       for (int i = lane; i < nq * NBX; i += 64) {
         const int lq = i / NBX, j = i - lq * NBX;
         const WideView &wv = s_view[q0 + lq];
