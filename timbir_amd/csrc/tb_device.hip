@@ -1009,6 +1009,10 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       const size_t lds = (size_t)st->max_nv * kHrStride * sizeof(float);
       if (lds + 2048 > st->lds_max) break;
       mw_fn fn = R == 1 ? k_icd_wide<17, 1, 8> : k_icd_wide<17, 2, 8>;
+      // A values from the stage's 17-bin column table when it fits in
+      // memory (the kernel evaluates them otherwise)
+      if (ensure_columns(st, 17) < 0) return -1;
+      P = st->kp();
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

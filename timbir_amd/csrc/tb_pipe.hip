@@ -1328,8 +1328,28 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wide(KP P, const int2 
         ld_r<R>(eb + (size_t)(ws + j) * P.mz, e[j]);
         ld_r<R>(wb + (size_t)(ws + j) * P.mz, w[j]);
       }
+      if (P.colt && P.colt_rec == col_rec(NBX)) {
+        // the stage's NBX-bin column table holds these values (same window,
+        // same arithmetic): read them while the (e, w) loads are in flight
+        constexpr int REC = col_rec(NBX);
+        const float4 *r4 = reinterpret_cast<const float4 *>(
+            P.colt + (((size_t)f * P.N_x * P.N_y + (size_t)yy * P.N_x + xx) * P.colt_nv +
+                      (act ? q : 0)) * REC);
+        float rr[REC];
 #pragma unroll
-      for (int j = 0; j < NBX; j++) a[j] = w0 >= 0 ? col_value(P, H, u, ws + j, loc, hic) : 0.0f;
+        for (int k = 0; k < REC / 4; k++) {
+          const float4 x4 = r4[k];
+          rr[4 * k] = x4.x;
+          rr[4 * k + 1] = x4.y;
+          rr[4 * k + 2] = x4.z;
+          rr[4 * k + 3] = x4.w;
+        }
+#pragma unroll
+        for (int j = 0; j < NBX; j++) a[j] = w0 >= 0 ? rr[j] : 0.0f;
+      } else {
+#pragma unroll
+        for (int j = 0; j < NBX; j++) a[j] = w0 >= 0 ? col_value(P, H, u, ws + j, loc, hic) : 0.0f;
+      }
       // theta sums (XT_VoxUpdate.c:61-92) with the generalized-Huber weights
       // (XT_VoxUpdate.c:66-80)
       double t1 = 0.0;
