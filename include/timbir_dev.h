@@ -66,6 +66,10 @@ typedef struct {
   int32_t init_fbp;               /* 1 = reconstruct() starts stage 0 from
                                      tb_init_fbp instead of zero (default 0,
                                      the reference's start) */
+  int32_t no_column_table;        /* 1 = build no A-column table: the
+                                     kernels that read it are skipped, the
+                                     wide-column and pass-based ones
+                                     evaluate the same values (default 0) */
 } tb_options;
 
 void tb_get_options(tb_options *opt);

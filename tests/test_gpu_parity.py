@@ -406,6 +406,47 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
     check_kernel(g, kernel)
 
 
+@pytest.mark.parametrize("mxy,mz,n_views,n_frames,kern", [
+    (1, 1, 2000, 1, 6),   # single-row passes, 3-bin table
+    (2, 2, 1200, 2, 6),   # 5-bin passes
+    (4, 4, 1400, 2, 7),   # 9-bin groups of passes
+    (8, 8, 600, 2, 7),    # 17-bin groups of passes
+    (8, 8, 256, 2, 5),    # wide columns, 17-bin table
+])
+def test_column_table_gives_the_same_bits(mxy, mz, n_views, n_frames, kern):
+    """The wide-column and pass-based kernels copy their A values from the
+    stage's column table, or evaluate them with the same arithmetic when
+    there is none (option no_column_table): two ICD iterations from the same
+    start leave bit-identical objects and error sinograms either way."""
+    n, nz = ((64, 16) if mxy == 8 else (64, 8) if mxy == 4 else (32, 4))
+    scan = scan_small(N=n, nz=nz, n_views=n_views, n_frames=n_frames, c_s=1.0, c_t=1.0)
+    stage = 1 if 1 < mxy < 8 else 0
+    old = tb.get_options()
+    out = []
+    try:
+        for no_table in (0, 1):
+            log(f"column table mxy={mxy} mz={mz} views={n_views} kernel={kern} "
+                f"no_column_table={no_table}")
+            tb.set_options(kernel=kern, no_column_table=no_table)
+            g = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=stage,
+                         prev_mult_z=2 * mz if stage else 0)
+            i = g.info
+            x = (np.random.default_rng(5).random((i["N_time"], i["N_z"], i["N_y"], i["N_x"]))
+                 * 0.02).astype(np.float32)
+            g.set_object(x)
+            g.init_error_sino()
+            for it in (1, 2):
+                g.icd_iteration(it)
+            st = g.stats()
+            assert st["global_launches"] == st["voxel_launches"] > 0, st
+            out.append((g.object(), g.error_sino()))
+            g.close()
+    finally:
+        tb.set_options(kernel=old.kernel, no_column_table=old.no_column_table)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+
+
 def test_fullsize_stage_matches_oracle():
     """The full 4D interlaced bench stage (256x256x8 slices x 16 frames, 1024
     views, 256 bins, zingers and rings, stiff prior): MAP cost after ICD

@@ -64,7 +64,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -859,12 +859,13 @@ static int ensure_order(tb_stage *st, int iter) {
 // (frame, pixel, view slot).  Returns 0 when the table is ready, 1 when it
 // would not fit (the caller then uses another kernel), -1 on a HIP error.
 static int ensure_columns(tb_stage *st, int nb) {
-  if (st->colt && st->colt_nb == nb) return 0;
+  if (st->colt && st->colt_nb == nb && !g_opt.no_column_table) return 0;
   if (st->colt) {
     HIPCHK(hipStreamSynchronize(st->stream));
     HIPCHK(hipFree(st->colt));
     st->colt = nullptr;
   }
+  if (g_opt.no_column_table) return 1;
   const int rec = col_rec(nb);
   const size_t n = (size_t)st->N_time * st->nxy() * st->max_nv * rec;
   size_t free_b = 0, total_b = 0;
