@@ -56,9 +56,10 @@ typedef struct {
                                      multi-wavefront kernels (columns of
                                      up to 17 bins, views in passes when
                                      there are many) where they apply,
-                                     6 = the pass-based ones alone, 7 =
-                                     the pass-based one that walks any
-                                     number of passes in groups */
+                                     6 = the pass kernel that holds its
+                                     passes in registers alone, 7 = the
+                                     pass kernel that walks any number of
+                                     passes in groups alone */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront
@@ -175,8 +176,21 @@ int tb_stage_run(tb_stage *st, int32_t max_iter, double *costs,
  * algorithmic bytes they moved (12 B per touched A-matrix entry: e read,
  * w read, e write).  Other kernels' time in other_ms.  lds_launches counts
  * the voxel-update launches that ran an LDS-resident kernel, global_launches
- * those that ran the multi-wavefront kernel with e in global memory (the
- * others ran the one-voxel-at-a-time kernel). */
+ * those that ran a multi-wavefront kernel with e in global memory (the
+ * others ran the one-voxel-at-a-time kernel); family_launches splits them
+ * by kernel.  The launch counters restart at tb_reset_stats, the table
+ * counters do not. */
+/* Voxel-update kernel families, the index of tb_stats.family_launches. */
+enum {
+  TB_K_ONE_VOXEL = 0, /* k_update_voxels: one voxel at a time */
+  TB_K_LDS = 1,       /* k_icd_lds: one wavefront per cell, (e, w) in LDS */
+  TB_K_MW = 2,        /* k_icd_mw: several wavefronts, (e, w) in LDS */
+  TB_K_MWG = 3,       /* k_icd_mwg: several wavefronts, e in global memory */
+  TB_K_WIDE = 4,      /* k_icd_wide: columns of up to 17 bins */
+  TB_K_WPASS = 5,     /* k_icd_wpass: views in passes held in registers */
+  TB_K_WLOOP = 6,     /* k_icd_wloop: views in groups of passes */
+  TB_KERNEL_FAMILIES = 8
+};
 typedef struct {
   double voxel_ms;
   double other_ms;
@@ -187,10 +201,89 @@ typedef struct {
   double alg_bytes;
   int64_t lds_launches;
   int64_t global_launches;
+  int64_t family_launches[TB_KERNEL_FAMILIES]; /* launches per TB_K_* */
+  int64_t column_builds; /* A-column tables built since the stage was created */
+  int32_t column_bins;   /* bins per record of the table held now (0: none) */
+  int32_t reserved;
 } tb_stats;
 int tb_get_stats(tb_stage *st, tb_stats *out);
 int tb_reset_stats(tb_stage *st);
 int tb_synchronize(tb_stage *st);
+
+/* Device address of the stage's object, float [N_time][N_z][N_y][N_x] in
+ * HBM (*bytes its size), valid until tb_stage_destroy.  For initialisers
+ * that write a starting object on the GPU without a host round trip, as
+ * initObject's initICD = 1 path reads one without interpolation
+ * (XT_ICD_update.c:558-571); work on the stage's data must be ordered after
+ * it with tb_synchronize. */
+int tb_stage_object_device(tb_stage *st, void **ptr, uint64_t *bytes);
+
+/* ---- learned initialiser: kernels of a 3D U-Net (tb_unet.hip) ----------
+ * No counterpart in the reference (its first stage starts from zero,
+ * XT_ICD_update.c:549-553).  Pointers are device addresses; `stream` is a
+ * hipStream_t (NULL: the default stream); every call is asynchronous on it.
+ * Activations and activation gradients are bf16 [n][d][h][w][c] (channels
+ * last, c a multiple of 32); convolution weights bf16 [27][cout][cin] with
+ * tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1); biases, weight gradients
+ * and optimiser state fp32.  Return 0, or -1 with tb_nn_last_error(). */
+const char *tb_nn_last_error(void);
+/* 3x3x3 convolution with zero padding on the matrix cores (bf16 x bf16,
+ * fp32 accumulate): out = [ReLU](bias + conv(in, w)); out fp32 when
+ * out_f32, else bf16.  bias may be NULL. */
+int tb_nn_conv3d(const void *in, const void *w, const float *bias, void *out, int32_t n,
+                 int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t relu,
+                 int32_t out_f32, void *stream);
+/* wt[t][ci][co] = w[26 - t][co][ci]: with it tb_nn_conv3d maps an output
+ * gradient to the input gradient. */
+int tb_nn_conv3d_flip(const void *w, void *wt, int32_t cin, int32_t cout, void *stream);
+/* gw[tap][co][ci] = sum over voxels of gout[v][co] in[v + off(tap)][ci]
+ * (fp32); part holds slices x 27 x cin x cout floats of partial sums, added
+ * in a fixed order. */
+int tb_nn_conv3d_wgrad(const void *in, const void *gout, float *gw, float *part,
+                       int32_t slices, int32_t n, int32_t d, int32_t h, int32_t wd,
+                       int32_t cin, int32_t cout, void *stream);
+/* out[c] = sum over nvox voxels of g[v][c] (bias gradient); part holds
+ * slices x c floats. */
+int tb_nn_channel_sum(const void *g, float *out, float *part, int32_t slices, int64_t nvox,
+                      int32_t c, void *stream);
+/* g[i] = 0 where the stored ReLU output y[i] is not positive. */
+int tb_nn_relu_bwd(const void *y, void *g, int64_t count, void *stream);
+/* 2x2x2 average pooling of [n][d][h][wd][c] (backward: in = the pooled
+ * gradient, out = the full-size gradient). */
+int tb_nn_avgpool2(const void *in, void *out, int32_t n, int32_t d, int32_t h, int32_t wd,
+                   int32_t c, int32_t backward, void *stream);
+/* out = [nearest x2 upsampling of a (ca channels), b (cb channels)] at size
+ * n x d x h x wd, and its backward (ga at half size, gb at full size). */
+int tb_nn_up2_concat(const void *a, const void *b, void *out, int32_t n, int32_t d, int32_t h,
+                     int32_t wd, int32_t ca, int32_t cb, void *stream);
+int tb_nn_up2_concat_bwd(const void *g, void *ga, void *gb, int32_t n, int32_t d, int32_t h,
+                         int32_t wd, int32_t ca, int32_t cb, void *stream);
+/* a[i] += b[i] (bf16). */
+int tb_nn_add(void *a, const void *b, int64_t count, void *stream);
+/* Network input from fp32 volumes: channel 0 = scale * fbp[v], channel 1 =
+ * scale * mean[v mod per_sample] (mean may be NULL), channels 2-31 zero. */
+int tb_nn_make_input(const float *fbp, const float *mean, void *x, int64_t nvox,
+                     int64_t per_sample, float scale, void *stream);
+/* mean[q] = (1 / nframes) sum_f x[f * per + q] (fp32). */
+int tb_nn_frame_mean(const float *x, float *mean, int64_t nframes, int64_t per, void *stream);
+/* Output head: out[v] = fbp[v] + (b + sum_c w[c] h[v][c]) / scale (fp32),
+ * and its backward (gh bf16; part = slices x (c + 1) partial sums of the
+ * weight and bias gradients, added with tb_nn_sum_parts). */
+int tb_nn_head(const void *h, const float *w, const float *b, const float *fbp, float *out,
+               int64_t nvox, int32_t c, float scale, void *stream);
+int tb_nn_head_bwd(const void *h, const float *w, const float *gout, void *gh, float *part,
+                   int32_t slices, int64_t nvox, int32_t c, float scale, void *stream);
+/* Mean squared error: gout[v] = 2 (out[v] - target[v]) / nvox, part[b] the
+ * fp64 sums of squares of `blocks` blocks. */
+int tb_nn_mse(const float *out, const float *target, float *gout, double *part, int32_t blocks,
+              int64_t nvox, void *stream);
+/* out[i] = sum over s < slices of part[s * count + i], in order. */
+int tb_nn_sum_parts(const float *part, float *out, int64_t count, int32_t slices, void *stream);
+/* Adam step `step` (from 1) on fp32 parameters p with gradient g and moments
+ * m, v; wb (may be NULL) receives the bf16 copy. */
+int tb_nn_adam(float *p, const float *g, float *m, float *v, void *wb, int64_t count, float lr,
+               float b1, float b2, float eps, int32_t step, void *stream);
+int tb_nn_to_bf16(const float *a, void *b, int64_t count, void *stream);
 
 #ifdef __cplusplus
 }

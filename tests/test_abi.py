@@ -34,11 +34,17 @@ def test_library_exports_every_declared_entry_point():
 
 
 def header_struct(name):
-    """(type, field) pairs of `typedef struct { ... } name;` in timbir_dev.h."""
+    """(type, field, count) of `typedef struct { ... } name;` in timbir_dev.h;
+    count is None for a scalar, else the array length."""
     text = open(os.path.join(ROOT, "include", "timbir_dev.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    consts = {k: int(v) for k, v in re.findall(r"\b(TB_\w+)\s*=\s*(\d+)", text)}
     body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + ";", text).group(1)
-    return [tuple(m.groups()) for m in re.finditer(r"(\w+)\s+(\w+);", body)]
+    out = []
+    for m in re.finditer(r"(\w+)\s+(\w+)(?:\[(\w+)\])?;", body):
+        t, f, n = m.groups()
+        out.append((t, f, None if n is None else int(consts.get(n, n))))
+    return out
 
 
 @pytest.mark.parametrize("name,cls", [("tb_options", "TbOptions"), ("tb_stats", "TbStats")])
@@ -49,8 +55,10 @@ def test_binding_structs_match_the_header(name, cls):
              "double": C.c_double}
     fields = header_struct(name)
     got = getattr(tb, cls)._fields_
-    assert [f for _, f in fields] == [f for f, _ in got]
-    assert [ctype[t] for t, _ in fields] == [t for _, t in got]
+    assert [f for _, f, _ in fields] == [f for f, _ in got]
+    want = [ctype[t] if n is None else ctype[t] * n for t, _, n in fields]
+    assert [C.sizeof(t) for t in want] == [C.sizeof(t) for _, t in got]
+    assert [getattr(t, "_type_", t) for t in want] == [getattr(t, "_type_", t) for _, t in got]
 
 
 def test_options_round_trip_without_gpu():

@@ -33,3 +33,52 @@ def test_concurrent_runs_equal_single_runs(tmp_path, monkeypatch, options):
         np.testing.assert_array_equal(a, b)
     monkeypatch.chdir(tmp_path)
     np.testing.assert_array_equal(tb.reconstruct(scans[1]), one[1])
+
+
+@pytest.mark.timeout(1200)
+def test_full_batched_workload(tmp_path, monkeypatch):
+    """The batched workload at its own sizes: 64 single-slice problems of
+    uneven size (128-512 px, 180-720 views, 4 detector rows each, I0 in
+    [5e3, 5e4]), reconstructed together, 3 ICD iterations per stage, NHICD
+    off.  Problems up to 192 px: the object against the CPU oracle's
+    reconstruct() on the same schedule (as close to the FP64 oracle as the
+    FP32 oracle is, within a factor 3 plus 1e-5 of the range: the
+    workload's stiff prior amplifies rounding).  Larger problems: the
+    batched object equals a run of that problem alone, bit for bit."""
+    from oracle import pyorc
+    o = tb.get_options()
+    saved = {k: getattr(o, k) for k, _ in o._fields_}
+    tb.set_options(max_iterations=3, verbose=0, no_nhicd=1)
+    try:
+        scans = datagen.batched()
+        print(f"batched workload: {len(scans)} problems", flush=True)
+        import time
+        t0 = time.time()
+        def done(k):
+            s = scans[k]
+            print(f"  problem {k} (N={s['proj_cols']}, {s['proj_num']} views) done at "
+                  f"{time.time() - t0:.1f} s", flush=True)
+        out = batch.reconstruct_many(scans, workers=8, progress=done)
+        print(f"  64 problems in {time.time() - t0:.1f} s", flush=True)
+        opt = dict(schedule=pyorc.SCHED_COLOUR, max_iterations=3, no_nhicd=1)
+        checked = 0
+        for k, (s, x) in enumerate(zip(scans, out)):
+            assert x.shape == (1, s["proj_rows"], s["proj_cols"], s["proj_cols"])
+            assert np.all(np.isfinite(x))
+            if s["proj_cols"] <= 192:
+                x32, _, _ = pyorc.reconstruct(s, pyorc.options(**opt))
+                x64, _, _ = pyorc.reconstruct(s, pyorc.options(**opt), precision="f64")
+                rel = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)) / (b.max() - b.min()))
+                r64, ref = rel(x, x64), rel(x32, x64)
+                print(f"  problem {k}: N={s['proj_cols']} views={s['proj_num']}: rmse/range "
+                      f"vs FP64 oracle {r64:.2e} (FP32 oracle {ref:.2e})", flush=True)
+                assert r64 <= 3 * ref + 1e-5
+                checked += 1
+        assert checked >= 20
+        for k in sorted(range(len(scans)), key=lambda q: -scans[q]["proj_cols"] * scans[q]["proj_num"])[:2]:
+            alone = batch.reconstruct_stages(scans[k])
+            print(f"  problem {k}: N={scans[k]['proj_cols']} views={scans[k]['proj_num']} "
+                  f"alone == batched", flush=True)
+            np.testing.assert_array_equal(alone, out[k])
+    finally:
+        tb.set_options(**saved)

@@ -284,3 +284,101 @@ def test_cost_file_holds_the_last_stage(tmp_path, monkeypatch, options):
     g1.close()
     np.testing.assert_array_equal(costs, c1)
     assert costs[-1] < costs[0]
+
+
+def oracle_pipeline(scan, sched, iters):
+    """The oracle's stages in reconstruct()'s order (XT_Engine.c:156-182),
+    colour schedule, NHICD off: each stage from the previous one's object,
+    map, offsets and variance, run to the reference's stop rule or iters.
+    Returns the per-stage ratio trajectories and the finest stage."""
+    ratios, prev, var = [], None, 1.0
+    for s, (mxy, mz) in enumerate(sched):
+        p = pyorc.Problem(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
+                          prev_mult_z=sched[s - 1][1] if s else 0, var_est=var,
+                          opt=pyorc.options(schedule=pyorc.SCHED_COLOUR, no_nhicd=1))
+        if prev is not None:
+            p.init_from_coarse(prev)
+            p.off()[...] = prev.off()
+        _, _, r = p.stage_run(iters)
+        ratios.append(r)
+        var, prev = p.var_est, p
+    return ratios, prev
+
+
+def test_stop_rule_fires_at_the_oracle_iteration(tmp_path, monkeypatch, options):
+    """reconstruct() ends every stage where the oracle does: the reference's
+    rule stops a stage after the first iteration past the first whose ratio
+    100 sum|dx| / sum|x| (last chunk) falls below convg_thresh
+    (XT_ICD_update.c:1096-1104, ICD_BackProject :1188-1195).  The threshold
+    is picked from the oracle's own trajectories so that no ratio of any
+    stage lies within 2% of it; the GPU's ratios agree with the oracle's to
+    about 1e-6, so the stop iterations must be equal.  Smooth prior, NHICD
+    off, ring correction and the zinger model on."""
+    import re
+    monkeypatch.chdir(tmp_path)
+    sched = [(2, 2), (1, 1)]
+    max_it = 40
+    base = scan4d()
+    chosen = None
+    for thresh in np.geomspace(40.0, 0.5, 25):
+        ratios, _ = oracle_pipeline(dict(base, convg_thresh=float(thresh)), sched, max_it)
+        stops = [len(r) for r in ratios]
+        near = min(np.min(np.abs(np.asarray(r) / thresh - 1)) for r in ratios)
+        if all(2 < n < max_it for n in stops) and near > 0.02:
+            chosen = (float(thresh), stops, ratios)
+            break
+    assert chosen, "no threshold with a clear margin"
+    thresh, stops, ratios = chosen
+    log(f"convg_thresh {thresh:.4f}: oracle stops after {stops} iterations, ratios "
+        f"{[np.round(r, 4).tolist() for r in ratios]}")
+    scan = dict(base, convg_thresh=thresh)
+    tb.set_options(max_iterations=max_it, verbose=1, no_nhicd=1)
+    dbg = tmp_path / "debug.log"
+    with open(dbg, "w") as fh:
+        x = tb.reconstruct(scan, debug=fh)
+    text = dbg.read_text()
+    got = [int(m.group(2)) for m in
+           re.finditer(r"Completed multi-resolution stage (\d+) after (\d+) iterations", text)]
+    log(f"  GPU stops after {got} iterations")
+    assert got == stops
+    _, fine = oracle_pipeline(scan, sched, max_it)
+    xo = fine.obj()
+    rmse = float(np.sqrt(np.mean((x - xo) ** 2)) / (xo.max() - xo.min()))
+    log(f"  rmse/range vs the oracle {rmse:.2e}")
+    assert rmse < 1e-5
+
+
+def test_converges_to_the_fp64_map_point_at_full_size(options):
+    """The plain 3D workload at its full size (128 x 128 x 4, 256 views,
+    c = 1e-2): reconstruct()'s stages on the GPU from zero, the finest stage
+    run on until its MAP cost is within 1e-5 of C*, the FP64 oracle's
+    converged cost (relative change below 1e-12, cached by
+    tests/golden/make_converged.py); the object must then be within 0.5% of
+    the range of x*, the oracle's converged image (RMSE)."""
+    import os
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "converged_plain3d.npz"))
+    scan = datagen.plain3d()
+    assert datagen.sha256(scan["projections"]) == str(gold["scan_sha256"]), "generator changed"
+    cstar, xstar = float(gold["cost"]), gold["x"]
+    tb.set_options(verbose=0)
+    g0 = tb.Stage(scan, mult_xy=2, mult_z=2, stage_idx=0)
+    n0 = len(g0.stage_run(0, with_cost=False)) - 1
+    g1 = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=2, var_est=g0.var_est)
+    g1.init_from_stage(g0)
+    g1.set_offset(g0.offset())
+    g0.close()
+    g1.init_error_sino()
+    gap = None
+    for it in range(1, 1001):
+        g1.icd_iteration(it)
+        c = g1.cost()[0]
+        gap = (c - cstar) / abs(cstar)
+        if abs(gap) <= 1e-5:
+            break
+    x = g1.object()[0]
+    rmse = float(np.sqrt(np.mean((x - xstar[0]) ** 2)) / (xstar.max() - xstar.min()))
+    log(f"  (2,2): {n0} iterations; finest: {it} iterations to |C - C*| = {abs(gap):.2e} |C*|, "
+        f"rmse/range {rmse:.2e} (oracle: {int(gold['iterations'])} iterations to 1e-12)")
+    assert abs(gap) <= 1e-5
+    assert rmse <= 5e-3
+    g1.close()

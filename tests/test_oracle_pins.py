@@ -332,24 +332,149 @@ def test_forward_projection_conserves_mass(mxy, mz):
     assert np.all(np.abs(ratio - 1.0) < 1e-2), (ratio.min(), ratio.max())
 
 
-def test_multires_upsampling_is_bilinear():
+def _clamped_bilinear_x2(coarse, axes):
+    """Upsample by 2 along each of `axes`: fine sample k sits at coarse
+    coordinate k/2 - 1/4 and takes the linear interpolant there, the edge
+    value beyond the first and last coarse centres (numpy's interp clamps)."""
+    out = np.asarray(coarse, np.float64)
+    for ax in axes:
+        n = out.shape[ax]
+        pos = np.arange(2 * n) / 2.0 - 0.25
+        out = np.apply_along_axis(lambda v: np.interp(pos, np.arange(n), v), ax, out)
+    return out
+
+
+@pytest.mark.parametrize("prev_mz", [1, 2])
+def test_multires_upsampling_is_clamped_bilinear(prev_mz):
+    """initObject's x2 upsample (XT_ICD_update.c:323-480), edges included
+    (:334-337, :345-348): every fine sample equals numpy's linear
+    interpolation at its centre, clamped to the edge value outside the
+    coarse centres, along x and y (initICD 2) and also z (initICD 3).  A
+    shifted sample position, a swapped 1/4-3/4 weight or an edge rule that
+    extrapolates fails it."""
     scan = small_scan(N=16, nz=4)
-    fine = pyorc.Problem(scan, stage_idx=1, mult_xy=1, mult_z=1, prev_mult_z=2)
-    coarse = pyorc.Problem(scan, stage_idx=0, mult_xy=2, mult_z=2)
-    i = coarse.info
-    zz, yy, xx = np.meshgrid(np.arange(i["N_z"]), np.arange(i["N_y"]), np.arange(i["N_x"]),
-                             indexing="ij")
-    ramp = (0.3 * xx + 0.2 * yy + 0.1 * zz).astype(np.float32)
-    coarse.obj()[...] = ramp[None]
+    coarse = pyorc.Problem(scan, stage_idx=0, mult_xy=2, mult_z=prev_mz, precision="f64")
+    fine = pyorc.Problem(scan, stage_idx=1, mult_xy=1, mult_z=1, prev_mult_z=prev_mz,
+                         precision="f64")
+    x = np.random.default_rng(21).random(coarse.obj_shape)
+    coarse.obj()[...] = x
     fine.init_from_coarse(coarse)
-    out = fine.obj()[0]
-    # interior of a linear ramp: bilinear x2 reproduces the ramp at the fine
-    # sample positions (c - 0.25, c + 0.25 in coarse units)
-    fz, fy, fx = np.meshgrid(np.arange(out.shape[0]), np.arange(out.shape[1]),
-                             np.arange(out.shape[2]), indexing="ij")
-    expect = 0.3 * (fx / 2 - 0.25) + 0.2 * (fy / 2 - 0.25) + 0.1 * (fz / 2 - 0.25)
-    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
-    np.testing.assert_allclose(out[inner], expect[inner], atol=1e-5)
+    axes = (3, 2) if prev_mz == 1 else (3, 2, 1)
+    expect = _clamped_bilinear_x2(x, axes)
+    np.testing.assert_allclose(fine.obj(), expect, rtol=0, atol=1e-13)
+    # the fine corner samples are the coarse corner samples, bit for bit
+    out = fine.obj()
+    zs = [(0, 0), (-1, -1)] if prev_mz == 2 else [(s, s) for s in range(x.shape[1])]
+    for yy, xx in ((0, 0), (0, -1), (-1, 0), (-1, -1)):
+        for zf, zc in zs:
+            np.testing.assert_array_equal(out[:, zf, yy, xx], x[:, zc, yy, xx])
+
+
+@pytest.mark.parametrize("prev_mz", [1, 2])
+def test_multires_upsampling_keeps_a_constant_field(prev_mz):
+    """A constant object and magnitude map come back constant everywhere,
+    edges and corners included, bit for bit in the float oracle."""
+    scan = small_scan(N=16, nz=4)
+    coarse = pyorc.Problem(scan, stage_idx=0, mult_xy=2, mult_z=prev_mz)
+    fine = pyorc.Problem(scan, stage_idx=1, mult_xy=1, mult_z=1, prev_mult_z=prev_mz)
+    coarse.obj()[...] = 0.375
+    coarse.mag()[...] = 1.25
+    fine.init_from_coarse(coarse)
+    assert np.all(fine.obj() == np.float32(0.375))
+    assert np.all(fine.mag() == np.float32(1.25))
+
+
+def test_variance_update_huber_branch_reaches_the_stationary_sigma():
+    """update_variance_parameter with outliers (XT_ICD_update.c:720-751,
+    quad_convex = 0): for a fixed error sinogram, repeated updates reach
+    the variance s that minimises the data cost
+        1/2 sum rho(e sqrt(w_raw / s)) + N/2 log s
+    with rho the generalized Huber function of computeCost
+    (XT_ICD_update.c:193-205).  Checked two ways: the stationarity
+    N s = sum_b e^2 w_raw + delta T sqrt(s) sum_not_b |e| sqrt(w_raw), solved
+    in closed form for the final selection b, and scipy's minimiser of the
+    cost over s with b re-evaluated at each trial s.  A dropped sqrt of the
+    old variance, a missing delta or T, or the quadratic branch used for the
+    outliers lands on another s."""
+    from scipy.optimize import minimize_scalar
+    p = _state(quad_convex=0, zingers=0.05)
+    T, dl = p.info["T"], p.info["delta"]
+    e = p.err().astype(np.float64).copy()
+    raw = p.wgt() * p.var_est
+    assert not p.sel().all(), "the scan must hold outliers"
+    last = None
+    for _ in range(200):
+        p.variance_update()
+        if last is not None and abs(p.var_est - last) <= 1e-15 * last:
+            break
+        last = p.var_est
+    s = p.var_est
+    np.testing.assert_array_equal(p.err(), e)  # e is not touched
+    np.testing.assert_allclose(p.wgt(), raw / s, rtol=1e-12)
+    b = np.abs(e * np.sqrt(raw / s)) < T
+    np.testing.assert_array_equal(p.sel().astype(bool), b)
+    assert 0 < (~b).sum() < b.size
+    N = e.size
+    A = np.sum(e[b] ** 2 * raw[b])
+    B = dl * T * np.sum(np.abs(e[~b]) * np.sqrt(raw[~b]))
+    root = ((B + np.sqrt(B * B + 4 * N * A)) / (2 * N)) ** 2
+    assert abs(s - root) <= 1e-10 * root
+
+    def cost(log_s):
+        u = e * np.sqrt(raw / np.exp(log_s))
+        rho = np.where(np.abs(u) < T, u * u, 2 * dl * T * np.abs(u) + T * T * (1 - 2 * dl))
+        return 0.5 * rho.sum() + 0.5 * N * log_s
+    r = minimize_scalar(cost, bracket=(np.log(s) - 0.5, np.log(s) + 0.5), tol=1e-12)
+    assert abs(np.exp(r.x) - s) <= 1e-6 * s
+
+
+@pytest.mark.parametrize("rings", [0, 2])
+def test_convergence_ratio_is_the_map_sum_over_the_last_chunk(rings):
+    """updateVoxelsTimeSlices' stop rule (XT_ICD_update.c:1066-1104): the
+    ratio is 100 x (sum over masked voxels of the magnitude map, i.e. of
+    |x_end - x_start| over the iteration when each voxel is visited once)
+    / (sum of |x| over the voxels updated in the last of the K chunks only,
+    because total_vox_mag restarts every chunk, :976).  The expected value
+    comes from object snapshots of a twin problem driven phase by phase;
+    the flag is ratio < convg_thresh.  Colour schedule, NHICD off, FP64."""
+    K = 10
+    scan = small_scan(N=16, nz=4, n_views=24, rings=0.01 if rings else 0.0,
+                      remove_rings=rings, quad_convex=1)
+    opt = pyorc.options(schedule=pyorc.SCHED_COLOUR, no_nhicd=1)
+    kw = dict(stage_idx=1, prev_mult_z=1) if rings else {}
+    a = pyorc.Problem(scan, opt=opt, precision="f64", **kw)
+    b = pyorc.Problem(scan, opt=opt, precision="f64", **kw)
+    mask = a.mask().astype(bool)
+    for p in (a, b):
+        p.init_error_sino()
+    b.L.orc_set_select_counts(b.h, b.info["U"])
+    ratios = []
+    for it in (1, 2, 3):
+        x_start = b.obj().copy()
+        for j in range(K):
+            if j == K - 1:
+                before = b.obj().copy()
+            for phase in (0, 1):
+                b.run_phase(phase, it, j, 1)
+                if rings:
+                    b.offset_update(rings)
+            b.vsc_select()
+        if b.info["updateVar"]:
+            b.variance_update()
+        x_end = b.obj().copy()
+        # the map is float (Real_arr_t), summed in double
+        num = np.abs(x_end - x_start)[:, :, mask].astype(np.float32).astype(np.float64).sum()
+        last = np.abs(x_end)[before != x_end].sum()
+        flag, ratio = a.icd_iteration(it)
+        np.testing.assert_array_equal(a.obj(), x_end)
+        assert abs(ratio - 100 * num / last) <= 1e-12 * ratio
+        assert flag == int(ratio < scan["convg_thresh"])
+        # the map holds |x_end - x_start| per voxel (one slice per block)
+        np.testing.assert_allclose(a.mag()[:, :, mask],
+                                   np.abs(x_end - x_start)[:, :, mask].astype(np.float32),
+                                   rtol=1e-6, atol=0)
+        ratios.append(ratio)
+    assert min(ratios) > 0
 
 
 def test_fullsize_fixture_matches_generator():
@@ -365,3 +490,18 @@ def test_fullsize_fixture_matches_generator():
     assert len(ref["costs"]) == ref["iterations"] == 3
     assert ref["costs"][0] > ref["costs"][1] > ref["costs"][2]
 
+
+
+def test_converged_fixture_matches_generator():
+    """The cached FP64-oracle MAP point of the plain 3D workload belongs to
+    the scan the generator makes today, and its cost trace decreases to a
+    relative change below 1e-12 (linear convergence: the gap left is of the
+    order of the last change)."""
+    g = np.load(os.path.join(GOLD, "converged_plain3d.npz"))
+    scan = datagen.plain3d()
+    assert datagen.sha256(scan["projections"]) == str(g["scan_sha256"])
+    t = g["trace"]
+    assert np.all(np.diff(t) < 0)
+    assert abs(t[-1] - t[-2]) / abs(t[-1]) < float(g["rel_stop"]) == 1e-12
+    assert float(g["cost"]) == t[-1]
+    assert g["x"].shape == (1, 4, 128, 128)

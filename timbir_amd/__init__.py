@@ -21,10 +21,16 @@ EXPORTS = [
     "tb_stage_destroy", "tb_stage_info", "tb_set_object", "tb_get_object", "tb_set_offset",
     "tb_get_offset", "tb_set_magmap", "tb_get_magmap", "tb_init_from_coarse", "tb_init_from_stage",
     "tb_get_error_sino", "tb_get_weights", "tb_get_detector_response", "tb_get_var_est",
-    "tb_init_error_sino", "tb_forward_project", "tb_init_fbp", "tb_cost", "tb_offset_update",
+    "tb_init_error_sino", "tb_forward_project", "tb_init_fbp",
+    "tb_cost", "tb_offset_update",
     "tb_variance_update", "tb_voxel_update", "tb_update_colour", "tb_vsc_select",
     "tb_nhicd_list", "tb_icd_iteration", "tb_stage_run", "tb_get_stats", "tb_reset_stats",
-    "tb_synchronize",
+    "tb_synchronize", "tb_stage_object_device",
+    # learned initialiser kernels (tb_unet.hip)
+    "tb_nn_last_error", "tb_nn_conv3d", "tb_nn_conv3d_flip", "tb_nn_conv3d_wgrad",
+    "tb_nn_channel_sum", "tb_nn_relu_bwd", "tb_nn_avgpool2", "tb_nn_up2_concat",
+    "tb_nn_up2_concat_bwd", "tb_nn_add", "tb_nn_make_input", "tb_nn_frame_mean", "tb_nn_head", "tb_nn_head_bwd",
+    "tb_nn_mse", "tb_nn_sum_parts", "tb_nn_adam", "tb_nn_to_bf16",
 ]
 
 
@@ -41,7 +47,14 @@ class TbStats(C.Structure):
                 ("voxel_launches", C.c_int64), ("updates", C.c_int64),
                 ("zero_skips", C.c_int64), ("entries", C.c_int64), ("alg_bytes", C.c_double),
                 ("lds_launches", C.c_int64),
-                ("global_launches", C.c_int64)]
+                ("global_launches", C.c_int64),
+                ("family_launches", C.c_int64 * 8),
+                ("column_builds", C.c_int64), ("column_bins", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+# voxel-update kernel families (TB_K_* in timbir_dev.h)
+K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP = range(7)
 
 
 class TimbirError(RuntimeError):
@@ -93,6 +106,7 @@ def lib() -> C.CDLL:
     L.tb_get_stats.argtypes = [vp, C.POINTER(TbStats)]
     L.tb_reset_stats.argtypes = [vp]
     L.tb_synchronize.argtypes = [vp]
+    L.tb_stage_object_device.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_uint64)]
     L.reconstruct.restype = C.c_int
     L.reconstruct.argtypes = ([C.POINTER(C.POINTER(C.c_float))] + [vp] * 5 + [i32] * 4
                               + [C.c_float] * 7 + [i32] * 2 + [C.c_float] * 2
@@ -272,13 +286,21 @@ class Stage:
     def stats(self) -> dict:
         s = TbStats()
         _check(self.L.tb_get_stats(self.h, C.byref(s)))
-        return {k: getattr(s, k) for k, _ in TbStats._fields_}
+        out = {k: getattr(s, k) for k, _ in TbStats._fields_}
+        out["family_launches"] = list(s.family_launches)
+        return out
 
     def reset_stats(self):
         _check(self.L.tb_reset_stats(self.h))
 
     def synchronize(self):
         _check(self.L.tb_synchronize(self.h))
+
+    def object_device(self):
+        """(device address, bytes) of the object in HBM."""
+        p, n = C.c_void_p(), C.c_uint64()
+        _check(self.L.tb_stage_object_device(self.h, C.byref(p), C.byref(n)))
+        return p.value, n.value
 
 
 def reconstruct(scan: dict, restart: int = 0, debug=None):

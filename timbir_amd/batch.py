@@ -15,7 +15,7 @@ call) bounds the useful number of workers.
 from __future__ import annotations
 
 import math
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import ThreadPoolExecutor, as_completed
 
 import numpy as np
 
@@ -66,10 +66,23 @@ def reconstruct_stages(scan: dict) -> np.ndarray:
                 h.close()
 
 
-def reconstruct_many(scans, workers: int = 4) -> list:
+def reconstruct_many(scans, workers: int = 4, progress=None) -> list:
     """reconstruct_stages() for every scan, `workers` scans at a time, each
-    in its own host thread with its own streams.  Results in input order."""
+    in its own host thread with its own streams.  Results in input order;
+    progress(k), when given, is called as scan k finishes."""
     if workers <= 1:
-        return [reconstruct_stages(s) for s in scans]
+        out = []
+        for k, s in enumerate(scans):
+            out.append(reconstruct_stages(s))
+            if progress:
+                progress(k)
+        return out
+    out = [None] * len(scans)
     with ThreadPoolExecutor(max_workers=workers) as pool:
-        return list(pool.map(reconstruct_stages, scans))
+        futs = {pool.submit(reconstruct_stages, s): k for k, s in enumerate(scans)}
+        for f in as_completed(futs):
+            k = futs[f]
+            out[k] = f.result()
+            if progress:
+                progress(k)
+    return out

@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtimbir_hip.so")
-SOURCES = ["tb_kernels.hip", "tb_pipe.hip", "tb_device.hip", "tb_engine.cpp"]
+SOURCES = ["tb_kernels.hip", "tb_pipe.hip", "tb_device.hip", "tb_unet.hip", "tb_engine.cpp"]
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
          "-Wno-unused-variable", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

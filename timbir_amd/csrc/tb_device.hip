@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -101,6 +102,7 @@ struct tb_stage {
   std::vector<float> angles;
   int max_nv;
   // device
+  int device = 0;
   hipStream_t stream = nullptr;
   float *e = nullptr, *w = nullptr, *y = nullptr, *d = nullptr, *obj = nullptr, *mag = nullptr;
   float *hr = nullptr, *filt = nullptr;
@@ -130,6 +132,8 @@ struct tb_stage {
   int ev_used = 0;
   double voxel_ms = 0, other_ms = 0;
   long long voxel_launches = 0, lds_launches = 0, global_launches = 0;
+  long long fam_launches[TB_KERNEL_FAMILIES] = {0};
+  long long colt_builds = 0;
   hipEvent_t o0 = nullptr, o1 = nullptr;
 
   KP kp() const {
@@ -369,7 +373,8 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
   }
   st->angles.assign(proj_angles, proj_angles + proj_num);
 
-  // device buffers
+  // device buffers, on the device current at creation
+  HIPCHK(hipGetDevice(&st->device));
   HIPCHK(hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking));
   const size_t ns = st->nsino(), no = st->nobj(), nxy = st->nxy();
   const int ncell = st->N_time * st->N_z;
@@ -718,45 +723,37 @@ static int harvest_events(tb_stage *st) {
 // ---------------------------------------------------------------------------
 // operations
 // ---------------------------------------------------------------------------
+// The slice-interleaved object copy the forward projector reads: one record
+// of kFwdZ floats per (frame, group of kFwdZ slices, pixel).
+static size_t fwd_records(const tb_stage *st) {
+  const int ngroups = (st->N_z + kFwdZ - 1) / kFwdZ;
+  return (size_t)st->N_time * ngroups * st->N_x * st->N_y;
+}
+
 // Forward projection into `out` (mode 0: A x, mode 1: the error sinogram):
-// the object is copied slice-interleaved into a temporary buffer, then
-// k_forward runs a block per (view, tile of kFwdT bins) x group of kFwdZ
-// slices.  Returns after the stream has finished.
-static int run_forward(tb_stage *st, float *out, int mode) {
+// the object is copied slice-interleaved into objt (2 float4 per record;
+// a temporary buffer when objt is null), then k_forward runs a block per
+// (view, tile of kFwdT bins) x group of kFwdZ slices.  Returns after the
+// stream has finished.
+static int run_forward(tb_stage *st, float *out, int mode, float4 *objt = nullptr) {
   KP P = st->kp();
   const int ngroups = (st->N_z + kFwdZ - 1) / kFwdZ;
   const int ntile = (st->N_r + kFwdT - 1) / kFwdT;
-  const size_t nrec = (size_t)st->N_time * ngroups * st->N_x * st->N_y;
-  float4 *objt = nullptr;
-  HIPCHK(dalloc(&objt, 2 * nrec));
+  const size_t nrec = fwd_records(st);
+  const bool own = objt == nullptr;
+  if (own) HIPCHK(dalloc(&objt, 2 * nrec));
   hipLaunchKernelGGL(k_fwd_interleave, dim3(grid_for(nrec)), dim3(256), 0, st->stream, P, objt);
   hipLaunchKernelGGL(k_forward, dim3(ntile * st->N_p, ngroups), dim3(kFwdT), 0, st->stream, P,
                      st->view_frame_d, objt, out, mode);
   hipError_t err = hipGetLastError();
   if (err == hipSuccess) err = hipStreamSynchronize(st->stream);
-  (void)hipFree(objt);
+  if (own) (void)hipFree(objt);
   HIPCHK(err);
   return 0;
 }
 
 extern "C" int tb_init_error_sino(tb_stage *st) {
   if (run_forward(st, st->e, 1)) return -1;
-  return 0;
-}
-
-extern "C" int tb_init_fbp(tb_stage *st) {
-  if ((size_t)st->N_r * sizeof(double) > 65536)
-    return fail("tb_init_fbp: %d detector bins exceed the filter's LDS row", st->N_r);
-  KP P = st->kp();
-  float *q = nullptr;
-  HIPCHK(dalloc(&q, (size_t)st->N_z * st->N_p * st->N_r));
-  hipLaunchKernelGGL(k_fbp_filter, dim3(st->N_p, st->N_z), dim3(256),
-                     st->N_r * sizeof(double), st->stream, P, q);
-  hipLaunchKernelGGL(k_fbp_back, dim3(grid_for(st->nobj())), dim3(256), 0, st->stream, P, q);
-  hipError_t err = hipGetLastError();
-  if (err == hipSuccess) err = hipStreamSynchronize(st->stream);
-  (void)hipFree(q);
-  HIPCHK(err);
   return 0;
 }
 
@@ -864,6 +861,7 @@ static int ensure_columns(tb_stage *st, int nb) {
     HIPCHK(hipStreamSynchronize(st->stream));
     HIPCHK(hipFree(st->colt));
     st->colt = nullptr;
+    st->colt_nb = 0;
   }
   if (g_opt.no_column_table) return 1;
   const int rec = col_rec(nb);
@@ -877,6 +875,7 @@ static int ensure_columns(tb_stage *st, int nb) {
     return 1;
   }
   st->colt_nb = nb;
+  st->colt_builds++;
   KP P = st->kp();
   const size_t recs = n / rec;
   const int grid = (int)std::min<size_t>((recs + 255) / 256, 65536);
@@ -928,6 +927,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
                        chunk, seq, count_mag, vpw);
     if (global) st->global_launches++;
     else st->lds_launches++;
+    st->fam_launches[global ? TB_K_MWG : TB_K_MW]++;
     launched = true;
     return 0;
   };
@@ -1020,6 +1020,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
                          chunk, seq, count_mag, mz / R);
       st->global_launches++;
+      st->fam_launches[TB_K_WIDE]++;
       launched = true;
     }
   }
@@ -1062,11 +1063,11 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
                          chunk, seq, count_mag, vpw);
       st->global_launches++;
+      st->fam_launches[TB_K_WPASS]++;
       launched = true;
     }
   }
-  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5 || g_opt.kernel == 6 ||
-                    g_opt.kernel == 7) &&
+  if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5 || g_opt.kernel == 7) &&
       (st->mult_z % 2 == 0 || st->mult_z == 1)) {
     // more passes than k_icd_wpass holds (the coarse stages of scans with
     // many views per frame): 15 theta waves walk their entries in groups
@@ -1092,6 +1093,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
                          chunk, seq, count_mag, vpw);
       st->global_launches++;
+      st->fam_launches[TB_K_WLOOP]++;
       launched = true;
     }
   }
@@ -1111,9 +1113,11 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       hipLaunchKernelGGL(fast, dim3(ncells), dim3(64), lds, st->stream, P, cells, iter, chunk,
                          seq, count_mag);
       st->lds_launches++;
+      st->fam_launches[TB_K_LDS]++;
       launched = true;
     }
   }
+  if (!launched) st->fam_launches[TB_K_ONE_VOXEL]++;
   if (launched) {
     // counted above
   } else if (st->max_nv <= 64)
@@ -1280,6 +1284,9 @@ extern "C" int tb_get_stats(tb_stage *st, tb_stats *out) {
   out->voxel_launches = st->voxel_launches;
   out->lds_launches = st->lds_launches;
   out->global_launches = st->global_launches;
+  for (int k = 0; k < TB_KERNEL_FAMILIES; k++) out->family_launches[k] = st->fam_launches[k];
+  out->column_builds = st->colt_builds;
+  out->column_bins = st->colt ? st->colt_nb : 0;
   out->alg_bytes = 12.0 * (double)out->entries;
   return 0;
 }
@@ -1300,6 +1307,14 @@ extern "C" int tb_reset_stats(tb_stage *st) {
   st->voxel_launches = 0;
   st->lds_launches = 0;
   st->global_launches = 0;
+  for (auto &k : st->fam_launches) k = 0;
+  return 0;
+}
+
+extern "C" int tb_stage_object_device(tb_stage *st, void **ptr, uint64_t *bytes) {
+  if (!st || !ptr) return fail("tb_stage_object_device: null argument");
+  *ptr = st->obj;
+  if (bytes) *bytes = (uint64_t)st->nobj() * sizeof(float);
   return 0;
 }
 
