@@ -124,6 +124,17 @@ double tb_get_var_est(const tb_stage *st);
 /* initErrorSinogam (XT_ICD_update.c:655-716): e = y - A x - d with the
  * forward projection of forward_project_voxel_AMat1D (XT_ForwardProject.c:8-25). */
 int tb_init_error_sino(tb_stage *st);
+/* tb_init_error_sino for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same result, bit for bit;
+ * calls no hipMalloc / hipFree: the forward projector's slice-interleaved
+ * object copy goes to a static per-GPU array of 64 MiB, so the stage must
+ * satisfy N_time x ceil(N_z / 8) x N_y x N_x x 32 B <= 64 MiB (-1 and an
+ * error message otherwise).  Runs on the device the stage was created on.
+ * Calls on one GPU share that array: calls that overlap in time on one GPU
+ * (stages on different streams or host threads) race and must be
+ * serialised by the caller. */
+int tb_init_error_sino_static(tb_stage *st);
 /* Forward projection alone (no y, no d): out = A x, [N_p][N_r][N_t]. */
 int tb_forward_project(tb_stage *st, float *out);
 /* Filtered back-projection of the stage's measurements into its object:

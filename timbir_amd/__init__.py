@@ -21,7 +21,7 @@ EXPORTS = [
     "tb_stage_destroy", "tb_stage_info", "tb_set_object", "tb_get_object", "tb_set_offset",
     "tb_get_offset", "tb_set_magmap", "tb_get_magmap", "tb_init_from_coarse", "tb_init_from_stage",
     "tb_get_error_sino", "tb_get_weights", "tb_get_detector_response", "tb_get_var_est",
-    "tb_init_error_sino", "tb_forward_project", "tb_init_fbp",
+    "tb_init_error_sino", "tb_init_error_sino_static", "tb_forward_project", "tb_init_fbp",
     "tb_cost", "tb_offset_update",
     "tb_variance_update", "tb_voxel_update", "tb_update_colour", "tb_vsc_select",
     "tb_nhicd_list", "tb_icd_iteration", "tb_stage_run", "tb_get_stats", "tb_reset_stats",

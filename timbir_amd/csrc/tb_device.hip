@@ -757,6 +757,67 @@ extern "C" int tb_init_error_sino(tb_stage *st) {
   return 0;
 }
 
+// Scratch of tb_init_error_sino_static: the interleaved object copy, in a
+// file-scope array so that the call allocates nothing.  Every GPU the
+// library runs on holds its own copy for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (the array's name in host code is not a device address).
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float4 scratch[kStaticScratchBytes / sizeof(float4)];
+static std::mutex g_scratch_mu;
+static float4 *g_scratch_dev[kMaxDevices];
+
+static int static_scratch(int dev, float4 **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail("static scratch: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail("static scratch: no address on device %d", dev);
+    g_scratch_dev[dev] = static_cast<float4 *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+extern "C" int tb_init_error_sino_static(tb_stage *st) {
+  const size_t need = 2 * fwd_records(st) * sizeof(float4);
+  if (need > kStaticScratchBytes)
+    return fail("tb_init_error_sino_static: the interleaved object takes %zu bytes "
+                "(%d frames x %d slices x %d x %d), more than the %zu-byte static scratch; "
+                "use tb_init_error_sino",
+                need, st->N_time, st->N_z, st->N_y, st->N_x, kStaticScratchBytes);
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != st->device) HIPCHK(hipSetDevice(st->device));
+  float4 *buf = nullptr;
+  int rc = static_scratch(st->device, &buf);
+  if (!rc) rc = run_forward(st, st->e, 1, buf);
+  if (cur != st->device) {
+    const std::string keep = g_err;
+    if (hipSetDevice(cur) != hipSuccess && !rc) rc = fail("tb_init_error_sino_static: restoring device %d", cur);
+    else if (rc) g_err = keep;
+  }
+  return rc;
+}
+
+extern "C" int tb_init_fbp(tb_stage *st) {
+  if ((size_t)st->N_r * sizeof(double) > 65536)
+    return fail("tb_init_fbp: %d detector bins exceed the filter's LDS row", st->N_r);
+  KP P = st->kp();
+  float *q = nullptr;
+  HIPCHK(dalloc(&q, (size_t)st->N_z * st->N_p * st->N_r));
+  hipLaunchKernelGGL(k_fbp_filter, dim3(st->N_p, st->N_z), dim3(256),
+                     st->N_r * sizeof(double), st->stream, P, q);
+  hipLaunchKernelGGL(k_fbp_back, dim3(grid_for(st->nobj())), dim3(256), 0, st->stream, P, q);
+  hipError_t err = hipGetLastError();
+  if (err == hipSuccess) err = hipStreamSynchronize(st->stream);
+  (void)hipFree(q);
+  HIPCHK(err);
+  return 0;
+}
+
 extern "C" int tb_forward_project(tb_stage *st, float *out) {
   float *tmp = nullptr;
   HIPCHK(dalloc(&tmp, st->nsino()));
