@@ -16,8 +16,12 @@
  *     the reference leaves it unset (XT_Engine.c:178-179).
  *   - MPI is gone: the whole scan is reconstructed by one process on one GPU,
  *     so the "nodes" of the reference's preconditions is 1.
- *   - restart=1 resumes after the last completed stage recorded in
- *     status.bin instead of re-running it.
+ *   - restart=1 re-runs the stage recorded in status.bin, as the
+ *     reference does (XT_Engine.c:143-157), but starts it from that stage's
+ *     own files (object_time_*.bin, mag_update.bin, proj_offset.bin,
+ *     variance_estimate.bin) as they are, where the reference interpolates
+ *     them as if they came from the stage before (a behavioural
+ *     difference: the reference's interpolation is mis-sized there).
  *
  * Link: -ltimbir_hip (libtimbir_hip.so) instead of -lmbir4d.
  */

@@ -71,6 +71,16 @@ typedef struct {
                                      kernels that read it are skipped, the
                                      wide-column and pass-based ones
                                      evaluate the same values (default 0) */
+  int32_t theta_fp;               /* theta1 / theta2 sums of the voxel
+                                     update (XT_VoxUpdate.c:61-92, double
+                                     in the reference): 0 = theta1 in FP64,
+                                     theta2 in FP32 (default); 1 = both in
+                                     FP32 with compensated (TwoSum)
+                                     accumulation; 2 = both in plain FP32.
+                                     1 and 2 exist in the multi-wavefront
+                                     global-e kernel for 3- and 5-bin
+                                     columns (the two finest stages); a
+                                     stage it does not serve fails */
 } tb_options;
 
 void tb_get_options(tb_options *opt);

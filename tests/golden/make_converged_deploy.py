@@ -1,0 +1,95 @@
+"""The converged MAP point of the deployment workload (datagen.deploy4d():
+512 x 512 x 32 slices x 16 frames, 2048 interlaced views, zingers and ring
+offsets, generalized-Huber data term and ring correction), for the
+time-to-converged measurement of bench.py.  Runs on the GPU (the FP64 CPU
+oracle would take days at this size).
+
+The run: reconstruct()'s stages from zero ((8,8), (4,4), (2,2)), each
+run on past the reference's stop rule until one ICD iteration changes its
+MAP cost by less than COARSE_REL (at most COARSE_CAP iterations: the
+coarse stages are cheap and carry the slow, temporally smooth part of the
+solution), then the finest stage until one ICD iteration changes the MAP
+cost (computeCost, XT_ICD_update.c:186-319, FP64 reductions) by less than
+REL_STOP of its value, at most CAP iterations.
+The same chain, held to the FP64 oracle's converged point, is
+tests/test_gpu_reconstruct.py::test_converges_to_the_fp64_map_point_at_full_size
+on the plain 3D workload.
+
+Kept: C* (the last cost), the cost trace, and x* sampled on a fixed grid
+(every 4th slice, every 8th row and column: 16 x 8 x 64 x 64 values, float16,
+whose rounding is far below the 0.5%-of-range RMSE bar) with the scan's
+SHA-256.
+
+    python tests/golden/make_converged_deploy.py OUT.npz
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import timbir_amd as tb  # noqa: E402
+from timbir_amd import datagen  # noqa: E402
+from timbir_amd.batch import multires_schedule  # noqa: E402
+
+REL_STOP = 1e-8
+CAP = 700
+COARSE_REL = 1e-6
+COARSE_CAP = 300
+SAMPLE = (slice(None), slice(None, None, 4), slice(None, None, 8), slice(None, None, 8))
+
+
+def main(out):
+    t0 = time.time()
+    scan = datagen.deploy4d()
+    sha = datagen.sha256(scan["projections"])
+    print(f"scan {time.time() - t0:.0f} s, sha {sha[:16]}", flush=True)
+    tb.set_options(verbose=0)
+    sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
+    prev, var = None, 1.0
+    for s, (mxy, mz) in enumerate(sched):
+        st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
+                      prev_mult_z=sched[s - 1][1] if s else 0, var_est=var)
+        if prev is not None:
+            st.init_from_stage(prev)
+            st.set_offset(prev.offset())
+            prev.close()
+        if s < len(sched) - 1:
+            st.init_error_sino()
+            c = [st.cost()[0]]
+            for it in range(1, COARSE_CAP + 1):
+                st.icd_iteration(it)
+                c.append(st.cost()[0])
+                if it > 2 and abs(c[-2] - c[-1]) < COARSE_REL * abs(c[-1]):
+                    break
+            print(f"stage ({mxy},{mz}): {len(c) - 1} iterations, cost {c[-1]:.10e} "
+                  f"({time.time() - t0:.0f} s)", flush=True)
+            prev, var = st, st.var_est
+    st.init_error_sino()
+    trace = [st.cost()[0]]
+    it = 0
+    for it in range(1, CAP + 1):
+        st.icd_iteration(it)
+        trace.append(st.cost()[0])
+        rel = abs(trace[-2] - trace[-1]) / abs(trace[-1])
+        if it % 10 == 0:
+            print(f"  it {it}: cost {trace[-1]:.12e} rel change {rel:.3e} "
+                  f"({time.time() - t0:.0f} s)", flush=True)
+        if rel < REL_STOP and it > 2:
+            break
+    x = st.object()
+    np.savez_compressed(out, cost=np.float64(trace[-1]), trace=np.asarray(trace),
+                        x_sample=x[SAMPLE].astype(np.float16), x_range=np.float64(x.max() - x.min()),
+                        x_sum=np.float64(x.astype(np.float64).sum()), iterations=np.int32(it),
+                        rel_stop=np.float64(REL_STOP), var_est=np.float64(st.var_est),
+                        scan_sha256=np.array(sha))
+    st.close()
+    print(f"finest: {it} iterations, C* = {trace[-1]:.12e} ({time.time() - t0:.0f} s)", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(HERE, "converged_deploy4d.npz"))

@@ -104,35 +104,52 @@ def test_reconstruct_equals_stage_pipeline(tmp_path, monkeypatch, options):
     g1.close()
 
 
-def test_restart_resumes_after_last_stage(tmp_path, monkeypatch, options):
-    """status.bin = 0 and the stage-0 files: restart runs only the finest stage
-    and ends where an uninterrupted run ends, bit for bit."""
+def test_restart_reruns_the_recorded_stage_from_its_files(tmp_path, monkeypatch, options):
+    """restart = 1 starts the stage loop again at the stage status.bin
+    records (XT_Engine.c:143-157), from that stage's own files as they are
+    (initObject's read without interpolation, XT_ICD_update.c:558-571), then
+    runs the later stages as usual: bit for bit the stage pipeline doing the
+    same.  A second restart (status.bin now 1) refines the last stage from
+    its files."""
     log("restart")
+    monkeypatch.chdir(tmp_path)
     scan = scan4d()
     tb.set_options(max_iterations=3, verbose=0)
-    full = tmp_path / "full"
-    part = tmp_path / "part"
-    full.mkdir()
-    part.mkdir()
-    monkeypatch.chdir(full)
-    x_full = tb.reconstruct(scan)
     # the files an interrupted run leaves after stage 0, from the stage API
-    monkeypatch.chdir(part)
     st = tb.Stage(scan, mult_xy=2, mult_z=2, stage_idx=0)
     st.stage_run(3)
-    obj, mag = st.object(), st.magmap()
+    obj0, off0, var0 = st.object(), st.offset(), st.var_est
     for f in range(scan["recon_num"]):
-        obj[f].tofile(f"object_time_{f}.bin")
-    mag.tofile("mag_update.bin")
-    st.offset().tofile("proj_offset.bin")
-    np.array([st.var_est], np.float64).tofile("variance_estimate.bin")
+        obj0[f].tofile(f"object_time_{f}.bin")
+    st.magmap().tofile("mag_update.bin")
+    off0.tofile("proj_offset.bin")
+    np.array([var0], np.float64).tofile("variance_estimate.bin")
     np.array([0], np.int32).tofile("status.bin")
     st.close()
     x_restart = tb.reconstruct(scan, restart=1)
-    np.testing.assert_array_equal(x_restart, x_full)
     assert np.fromfile("status.bin", np.int32)[0] == 1
-    # every stage done: restart returns the last stage's object
-    np.testing.assert_array_equal(tb.reconstruct(scan, restart=1), x_full)
+    # the same through the stage API
+    g0 = tb.Stage(scan, mult_xy=2, mult_z=2, stage_idx=0, var_est=var0)
+    g0.set_object(obj0)
+    g0.set_offset(off0)
+    g0.stage_run(3, with_cost=False)
+    g1 = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=2, var_est=g0.var_est)
+    g1.init_from_stage(g0)
+    g1.set_offset(g0.offset())
+    g0.close()
+    g1.stage_run(3, with_cost=False)
+    np.testing.assert_array_equal(x_restart, g1.object())
+    obj1, mag1, off1, var1 = g1.object(), g1.magmap(), g1.offset(), g1.var_est
+    g1.close()
+    # every stage done: the last one again, from its files
+    x_again = tb.reconstruct(scan, restart=1)
+    g2 = tb.Stage(scan, mult_xy=1, mult_z=1, stage_idx=1, prev_mult_z=2, var_est=var1)
+    g2.set_object(obj1)
+    g2.set_magmap(mag1)
+    g2.set_offset(off1)
+    g2.stage_run(3, with_cost=False)
+    np.testing.assert_array_equal(x_again, g2.object())
+    g2.close()
 
 
 def test_bin_data_driver_matches_library(tmp_path, monkeypatch, options):

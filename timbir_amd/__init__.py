@@ -39,7 +39,8 @@ class TbOptions(C.Structure):
                 ("max_iterations", C.c_int32), ("no_nhicd", C.c_int32),
                 ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32),
                 ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
-                ("init_fbp", C.c_int32), ("no_column_table", C.c_int32)]
+                ("init_fbp", C.c_int32), ("no_column_table", C.c_int32),
+                ("theta_fp", C.c_int32)]
 
 
 class TbStats(C.Structure):

@@ -30,7 +30,7 @@ template <int VPL, int NB, int MZ, int G>
 __global__ void k_icd_lds(KP, const int2 *, int, int, int, int);
 template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mw(KP, const int2 *, int, int, int, int, int);
-template <int NB, int MZ, int VPL, int MAXTH>
+template <int NB, int MZ, int VPL, int MAXTH, int TF>
 __global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int MAXTH>
 __global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
@@ -65,7 +65,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -992,7 +992,10 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     launched = true;
     return 0;
   };
-  if (nb && (g_opt.kernel == 0 || g_opt.kernel == 3)) {
+  // FP32 theta sums (theta_fp 1 or 2) exist in the global-e kernel only
+  const int tf = g_opt.theta_fp;
+  if (tf < 0 || tf > 2) return fail("theta_fp %d: 0, 1 or 2", tf);
+  if (nb && tf == 0 && (g_opt.kernel == 0 || g_opt.kernel == 3)) {
     // one wavefront per SIMD where the views allow: at most 3 theta waves
     // (one view per lane, else two), else one view per lane and more waves
     const int vw = 64 / (nb * st->mult_z);
@@ -1029,21 +1032,36 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     // instantiations: (NB, MZ, VPL) -> the widest MAXTH that keeps the
     // registers unspilled (15, or 11 with four views per lane)
     auto pick = [&](int vpl, int nth) -> mw_fn {
+      if (tf != 0) {
+        // the FP32 theta variants: 3- and 5-bin columns
+        if (gnb == 3 && mz == 1) {
+          if (vpl == 1) return nth <= 8 ? (tf == 1 ? k_icd_mwg<3, 1, 1, 8, 1> : k_icd_mwg<3, 1, 1, 8, 2>)
+                                        : (tf == 1 ? k_icd_mwg<3, 1, 1, 15, 1> : k_icd_mwg<3, 1, 1, 15, 2>);
+          if (vpl == 2) return tf == 1 ? k_icd_mwg<3, 1, 2, 15, 1> : k_icd_mwg<3, 1, 2, 15, 2>;
+          if (vpl == 4 && nth <= 11) return tf == 1 ? k_icd_mwg<3, 1, 4, 11, 1> : k_icd_mwg<3, 1, 4, 11, 2>;
+        } else if (gnb == 5 && mz == 2) {
+          if (vpl == 1) return nth <= 8 ? (tf == 1 ? k_icd_mwg<5, 2, 1, 8, 1> : k_icd_mwg<5, 2, 1, 8, 2>)
+                                        : (tf == 1 ? k_icd_mwg<5, 2, 1, 15, 1> : k_icd_mwg<5, 2, 1, 15, 2>);
+          if (vpl == 2) return tf == 1 ? k_icd_mwg<5, 2, 2, 15, 1> : k_icd_mwg<5, 2, 2, 15, 2>;
+          if (vpl == 4 && nth <= 11) return tf == 1 ? k_icd_mwg<5, 2, 4, 11, 1> : k_icd_mwg<5, 2, 4, 11, 2>;
+        }
+        return nullptr;
+      }
       if (gnb == 3 && mz == 1) {
-        if (vpl == 1) return nth <= 8 ? k_icd_mwg<3, 1, 1, 8> : k_icd_mwg<3, 1, 1, 15>;
-        if (vpl == 2) return k_icd_mwg<3, 1, 2, 15>;
-        if (vpl == 4 && nth <= 11) return k_icd_mwg<3, 1, 4, 11>;
+        if (vpl == 1) return nth <= 8 ? k_icd_mwg<3, 1, 1, 8, 0> : k_icd_mwg<3, 1, 1, 15, 0>;
+        if (vpl == 2) return k_icd_mwg<3, 1, 2, 15, 0>;
+        if (vpl == 4 && nth <= 11) return k_icd_mwg<3, 1, 4, 11, 0>;
       } else if (gnb == 5 && mz == 2) {
-        if (vpl == 1) return nth <= 8 ? k_icd_mwg<5, 2, 1, 8> : k_icd_mwg<5, 2, 1, 15>;
-        if (vpl == 2) return k_icd_mwg<5, 2, 2, 15>;
-        if (vpl == 4 && nth <= 11) return k_icd_mwg<5, 2, 4, 11>;
+        if (vpl == 1) return nth <= 8 ? k_icd_mwg<5, 2, 1, 8, 0> : k_icd_mwg<5, 2, 1, 15, 0>;
+        if (vpl == 2) return k_icd_mwg<5, 2, 2, 15, 0>;
+        if (vpl == 4 && nth <= 11) return k_icd_mwg<5, 2, 4, 11, 0>;
       } else if (gnb == 9 && mz == 4) {
-        if (vpl == 1) return k_icd_mwg<9, 4, 1, 15>;
-        if (vpl == 2) return k_icd_mwg<9, 4, 2, 15>;
+        if (vpl == 1) return k_icd_mwg<9, 4, 1, 15, 0>;
+        if (vpl == 2) return k_icd_mwg<9, 4, 2, 15, 0>;
       } else if (gnb == 9 && mz == 2) {
-        if (vpl == 1) return k_icd_mwg<9, 2, 1, 15>;
-        if (vpl == 2) return k_icd_mwg<9, 2, 2, 15>;
-        if (vpl == 4 && nth <= 11) return k_icd_mwg<9, 2, 4, 11>;
+        if (vpl == 1) return k_icd_mwg<9, 2, 1, 15, 0>;
+        if (vpl == 2) return k_icd_mwg<9, 2, 2, 15, 0>;
+        if (vpl == 4 && nth <= 11) return k_icd_mwg<9, 2, 4, 11, 0>;
       }
       return nullptr;
     };
@@ -1058,6 +1076,9 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     }
     if (fn && run_mw(fn, gnb, nth, vpw, 0, true) < 0) return -1;
   }
+  if (!launched && tf != 0)
+    return fail("theta_fp %d: FP32 theta sums exist in the multi-wavefront global-e kernel "
+                "with 3- or 5-bin columns only, which does not serve this stage", tf);
   // kernel 6 asks for the pass form alone
   const bool wpass_first = g_opt.kernel == 6;
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5) && 2 * st->mult_xy + 1 <= 17 &&

@@ -190,6 +190,69 @@ __device__ __forceinline__ void wave_total_df(double &d, float &f) {
   d = swap_sum_d<true>(d);
   f = swap_sum_f<true>(f);
 }
+// Two float sums in plain FP32, level by level as wave_total_df.
+__device__ __forceinline__ void wave_total_ff(float &a, float &b) {
+  a += dpp_f<0xB1>(a);
+  b += dpp_f<0xB1>(b);
+  a += dpp_f<0x4E>(a);
+  b += dpp_f<0x4E>(b);
+  a += dpp_f<0x141>(a);
+  b += dpp_f<0x141>(b);
+  a += dpp_f<0x140>(a);
+  b += dpp_f<0x140>(b);
+  a = swap_sum_f<false>(a);
+  b = swap_sum_f<false>(b);
+  a = swap_sum_f<true>(a);
+  b = swap_sum_f<true>(b);
+}
+
+// Error-free TwoSum of a and b: a + b = s + err exactly (Knuth).
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &err) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const float bp = s - a;
+  err = (a - (s - bp)) + (b - bp);
+}
+// One level of a compensated pair tree: (s, c) += (so, co).
+__device__ __forceinline__ void cc_level(float &s, float &c, float so, float co) {
+  float t, err;
+  two_sum(s, so, t, err);
+  s = t;
+  c = c + co + err;
+}
+template <int CTRL>
+__device__ __forceinline__ void cc_dpp(float &s, float &c) {
+  cc_level(s, c, dpp_f<CTRL>(s), dpp_f<CTRL>(c));
+}
+template <bool HALVES>
+__device__ __forceinline__ void cc_swap(float &s, float &c) {
+  const unsigned us = __float_as_uint(s), uc = __float_as_uint(c);
+  const auto rs = HALVES ? __builtin_amdgcn_permlane32_swap(us, us, false, false)
+                         : __builtin_amdgcn_permlane16_swap(us, us, false, false);
+  const auto rc = HALVES ? __builtin_amdgcn_permlane32_swap(uc, uc, false, false)
+                         : __builtin_amdgcn_permlane16_swap(uc, uc, false, false);
+  s = __uint_as_float(rs[0]);
+  c = __uint_as_float(rc[0]);
+  cc_level(s, c, __uint_as_float(rs[1]), __uint_as_float(rc[1]));
+}
+// Two compensated FP32 sums (s1, c1) and (s2, c2) over the wavefront: the
+// same DPP / permlane tree as wave_total, each level a TwoSum whose rounding
+// error joins the compensation term; the totals end in every lane.
+__device__ __forceinline__ void wave_total_cc(float &s1, float &c1, float &s2, float &c2) {
+  cc_dpp<0xB1>(s1, c1);
+  cc_dpp<0xB1>(s2, c2);
+  cc_dpp<0x4E>(s1, c1);
+  cc_dpp<0x4E>(s2, c2);
+  cc_dpp<0x141>(s1, c1);
+  cc_dpp<0x141>(s2, c2);
+  cc_dpp<0x140>(s1, c1);
+  cc_dpp<0x140>(s2, c2);
+  cc_swap<false>(s1, c1);
+  cc_swap<false>(s2, c2);
+  cc_swap<true>(s1, c1);
+  cc_swap<true>(s2, c2);
+}
+
 // Two float sums over lanes 0..31 only (rows 0 and 1), taken level by level;
 // the totals end in every lane of those rows.
 __device__ __forceinline__ void lo32_total_ff(float &a, float &b) {

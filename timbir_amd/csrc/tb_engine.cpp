@@ -134,9 +134,11 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
   }
   log_info(dbg, verbose, "Number of multi-resolution stages is %d.\n", nres);
 
-  // resume after the last completed stage (status.bin), reading the files
-  // that stage wrote
+  // restart (XT_Engine.c:143-157): the loop starts again at the stage
+  // status.bin records, from that stage's own files read without
+  // interpolation (initObject's initICD = 1 path, XT_ICD_update.c:558-571)
   int first = 0;
+  bool resume = false;
   std::vector<float> prev_obj, prev_mag, prev_off;
   double var_est = 1.0;  // VAR_PARAM_INIT
   int prev_nz = 0;
@@ -146,7 +148,8 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
       log_err(dbg, "Cannot read a valid status.bin for restart.\n");
       return -1;
     }
-    first = last + 1;
+    first = last;
+    resume = true;
     {
       // the files of the last completed stage: the next stage starts from
       // them, or they are the result when every stage is done
@@ -233,10 +236,11 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
       tb_stage_destroy(prev_st);
       prev_st = nullptr;
       if (!rc && !prev_off.empty()) rc = tb_set_offset(st, prev_off.data());
-    } else if (s > first || (s == first && !prev_obj.empty())) {
-      // restart: the previous stage's files
-      rc = tb_init_from_coarse(st, prev_obj.data(), prev_nz, prev_mag.data());
-      if (!rc && !prev_off.empty()) rc = tb_set_offset(st, prev_off.data());
+    } else if (resume && s == first) {
+      // restart: this stage again, from its own files as they are
+      rc = tb_set_object(st, prev_obj.data());
+      if (!rc && s > 0) rc = tb_set_magmap(st, prev_mag.data());
+      if (!rc) rc = tb_set_offset(st, prev_off.data());
     } else if (opt.init_fbp) {
       // stage 0 from the filtered back-projection instead of zero
       rc = tb_init_fbp(st);

@@ -27,6 +27,8 @@
 // wave loads the 28 neighbour values one group of voxels ahead and patches
 // in-flight copies when a neighbour is updated in between.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 #include <math.h>
 #include "tb_common.h"
 #include "tb_devutil.h"
@@ -413,17 +415,23 @@ struct StepIn {
   unsigned long long nent[16];
 };
 
-template <int MAXTH>
+// How the theta sums are formed (tb_options.theta_fp): in every mode each
+// lane's products are FP32.  TF 0: theta1 summed in FP64 over lanes and
+// waves, theta2 in FP32.  TF 1: both in FP32 with compensated (TwoSum)
+// accumulation over lanes; the waves' compensated totals s + c are exact in
+// FP64 and added there.  TF 2: both in plain FP32 throughout.
+template <int MAXTH, int TF = 0>
 __device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int par, int nth,
                                             float *V_out, bool *skip_out) {
 #pragma clang fp contract(off)
   // every slot is read at once (one LDS latency) and added as a fixed
   // pairwise tree over the MAXTH slots; slots of absent waves hold zero
-  double a1[MAXTH];
+  using T1T = typename std::conditional<TF == 2, float, double>::type;
+  T1T a1[MAXTH];
   float a2[MAXTH];
 #pragma unroll
   for (int w = 0; w < MAXTH; w++) {
-    a1[w] = S.t1[par][w];
+    a1[w] = (T1T)S.t1[par][w];
     a2[w] = S.t2[par][w];
   }
 #pragma unroll
@@ -435,7 +443,7 @@ __device__ __forceinline__ float solve_step(const KP &P, const StepIn &S, int pa
       a2[w] = a2[w] + a2[w + h];
     }
   }
-  const double T1 = a1[0];
+  const double T1 = (double)a1[0];
   const float T2 = a2[0];
   (void)nth;
   const float pb = S.pr[par][0], pbx = S.pr[par][1], V = S.pr[par][2];
@@ -1026,7 +1034,7 @@ __device__ __forceinline__ void st_rows(float *p, const float (&o)[MZ]) {
   }
 }
 
-template <int NB, int MZ, int VPL, int MAXTH>
+template <int NB, int MZ, int VPL, int MAXTH, int TF>
 __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *__restrict__ cells, int iter,
                                                   int chunk, int seq, int count_mag, int vpw) {
   constexpr int G = 8;             // pixels read per list load
@@ -1167,21 +1175,54 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *
         }
         double t1;
         float t2;
+        if constexpr (TF == 0) {
 #pragma unroll
-        for (int k = 0; k < VPL; k++) {
+          for (int k = 0; k < VPL; k++) {
 #pragma unroll
-          for (int m = 0; m < MZ; m++) {
-            const float e = ce[k][m];
-            const float cw = cwk[k][m];
-            const float ac = ca[k] * cw;
-            const double p1 = (double)(-e * ac);
-            const float p2 = ca[k] * ac;
-            t1 = (k == 0 && m == 0) ? p1 : t1 + p1;
-            t2 = (k == 0 && m == 0) ? p2 : t2 + p2;
+            for (int m = 0; m < MZ; m++) {
+              const float e = ce[k][m];
+              const float cw = cwk[k][m];
+              const float ac = ca[k] * cw;
+              const double p1 = (double)(-e * ac);
+              const float p2 = ca[k] * ac;
+              t1 = (k == 0 && m == 0) ? p1 : t1 + p1;
+              t2 = (k == 0 && m == 0) ? p2 : t2 + p2;
+            }
+            nent += ca[k] > 0.0f ? MZ : 0;
           }
-          nent += ca[k] > 0.0f ? MZ : 0;
+          wave_total_df(t1, t2);
+        } else {
+          // FP32 sums: compensated (TF 1) or plain (TF 2)
+          float s1 = 0.f, c1 = 0.f, s2 = 0.f, c2 = 0.f;
+#pragma unroll
+          for (int k = 0; k < VPL; k++) {
+#pragma unroll
+            for (int m = 0; m < MZ; m++) {
+              const float e = ce[k][m];
+              const float cw = cwk[k][m];
+              const float ac = ca[k] * cw;
+              const float p1 = -e * ac;
+              const float p2 = ca[k] * ac;
+              if constexpr (TF == 1) {
+                cadd(s1, c1, p1);
+                cadd(s2, c2, p2);
+              } else {
+                s1 += p1;
+                s2 += p2;
+              }
+            }
+            nent += ca[k] > 0.0f ? MZ : 0;
+          }
+          if constexpr (TF == 1) {
+            wave_total_cc(s1, c1, s2, c2);
+            t1 = (double)s1 + (double)c1;
+            t2 = s2 + c2;
+          } else {
+            wave_total_ff(s1, s2);
+            t1 = (double)s1;
+            t2 = s2;
+          }
         }
-        wave_total_df(t1, t2);
         if (lane == 0) {
           S.t1[par][wave] = t1;
           S.t2[par][wave] = t2;
@@ -1198,7 +1239,7 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *
         __syncthreads();
         float V;
         bool skip;
-        const float xn = solve_step<MAXTH>(P, S, par, nth, &V, &skip);
+        const float xn = solve_step<MAXTH, TF>(P, S, par, nth, &V, &skip);
         const float dx = xn - V;
         wait_vm_all();
 #pragma unroll
@@ -1852,21 +1893,40 @@ template __global__ void k_icd_wpass<5, 2, 16, 11, true>(KP, const int2 *, int, 
 template __global__ void k_icd_wpass<3, 1, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_wpass<3, 1, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
 
-template __global__ void k_icd_mwg<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<5, 2, 1, 8>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<5, 2, 1, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<9, 4, 1, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<9, 4, 2, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 4, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 4, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
 // more views per lane for cells with many views (single-frame scans); with
 // 4 views per lane the registers fit only 11 theta waves without spilling
-template __global__ void k_icd_mwg<3, 1, 2, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<3, 1, 4, 11>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<5, 2, 2, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<5, 2, 4, 11>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<9, 2, 1, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<9, 2, 2, 15>(KP, const int2 *, int, int, int, int, int);
-template __global__ void k_icd_mwg<9, 2, 4, 11>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 4, 11, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 4, 11, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 2, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 2, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<9, 2, 4, 11, 0>(KP, const int2 *, int, int, int, int, int);
+
+// FP32 theta sums, compensated (TF 1) and plain (TF 2), for the 3- and 5-bin
+// columns (the finest and second-finest stages)
+template __global__ void k_icd_mwg<3, 1, 1, 8, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 1, 15, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 2, 15, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 4, 11, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 8, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 15, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 2, 15, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 4, 11, 1>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 1, 8, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 1, 15, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 2, 15, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<3, 1, 4, 11, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 8, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 1, 15, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 2, 15, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg<5, 2, 4, 11, 2>(KP, const int2 *, int, int, int, int, int);
 
 // finest stage (m_xy = m_z = 1: at most 3 bins per view, one detector row)
 template __global__ void k_icd_lds<1, 3, 1, 4>(KP, const int2 *, int, int, int, int);

@@ -55,7 +55,7 @@ def nn_lib():
         "tb_nn_conv3d_wgrad": [vp, vp, vp, vp] + [i32] * 7 + [vp],
         "tb_nn_channel_sum": [vp, vp, vp, i32, i64, i32, vp],
         "tb_nn_relu_bwd": [vp, vp, i64, vp],
-        "tb_nn_avgpool2": [vp, vp] + [i32] * 7 + [vp],
+        "tb_nn_avgpool2": [vp, vp] + [i32] * 6 + [vp],
         "tb_nn_up2_concat": [vp, vp, vp] + [i32] * 6 + [vp],
         "tb_nn_up2_concat_bwd": [vp, vp, vp] + [i32] * 6 + [vp],
         "tb_nn_add": [vp, vp, i64, vp],

@@ -88,10 +88,14 @@ def synth_scan(seed: int) -> dict:
     return datagen._scan(proj, wts, angles, times, rt, NZ, N, FRAMES, truth, **PRIOR)
 
 
-def fbp_and_target(scan: dict, finest_iters: int) -> tuple:
+def fbp_and_target(scan: dict, finest_iters: int, coarse_rel: float = 1e-5,
+                   coarse_cap: int = 200) -> tuple:
     """The filtered back-projection of the scan at the finest resolution and
-    the MBIR object after reconstruct()'s stages with the finest stage run
-    finest_iters ICD iterations (both [FRAMES][NZ][N][N])."""
+    its MBIR object (both [FRAMES][NZ][N][N]): reconstruct()'s stages, the
+    coarse ones run on until an iteration changes their MAP cost by less
+    than coarse_rel (at most coarse_cap iterations; they are cheap and carry
+    the slow, temporally smooth part of the solution), then finest_iters
+    ICD iterations of the finest stage."""
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     f = Stage(scan, mult_xy=1, mult_z=1, stage_idx=len(sched) - 1,
               prev_mult_z=sched[-2][1] if len(sched) > 1 else 0)
@@ -107,7 +111,14 @@ def fbp_and_target(scan: dict, finest_iters: int) -> tuple:
             st.set_offset(prev.offset())
             prev.close()
         if s < len(sched) - 1:
-            st.stage_run(0, with_cost=False)
+            st.init_error_sino()
+            last = st.cost()[0]
+            for it in range(1, coarse_cap + 1):
+                st.icd_iteration(it)
+                c = st.cost()[0]
+                if it > 2 and abs(last - c) < coarse_rel * abs(c):
+                    break
+                last = c
             prev, var = st, st.var_est
         else:
             st.init_error_sino()
