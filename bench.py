@@ -1,17 +1,27 @@
 """Benchmark of the MBIR_4D ICD/NHICD hot path on one MI355X.
 
-Workload (one GPU): 4D interlaced-view MBIR, finest multi-resolution stage of
-a 256 x 256 x 8-slice Cahn-Hilliard microstructure over 16 time frames, 1024
-interlaced views (K = 8 sub-frames) over one half turn, 256 detector bins,
+Workload (one GPU): the deployment scan, 4D interlaced-view MBIR of a
+512 x 512 x 32-slice Cahn-Hilliard microstructure over 16 time frames, 2048
+interlaced views (K = 8 sub-frames) over one half turn, 512 detector bins,
 Poisson I0 = 1e4, zingers in 0.1% of the bins, per-detector ring offsets
 N(0, 0.01); generalized-Huber data term, ring correction and variance
-estimation on.  A step is one full ICD iteration of that stage (HICD + NHICD
-colour phases, offset solves after every phase, VSC selection, variance
-update, convergence check), inputs resident in HBM.
+estimation on (datagen.deploy4d).
 
-Prints one JSON line.  ``value`` is voxel updates per second over the timed
-steps (wall clock, all kernels included); the roofline block covers the
-voxel-update kernel alone (HIP events on the library's stream).
+A step is one full ICD iteration of its finest multi-resolution stage (HICD +
+NHICD colour phases, offset solves after every phase, VSC selection, variance
+update, convergence check), inputs resident in HBM.  ``value`` is voxel
+updates per second over the timed steps (wall clock, all kernels included);
+the roofline block covers the voxel-update kernel alone (HIP events on the
+library's stream).
+
+``time_to_converged`` times the three starts of timbir_amd.converge (zero and
+filtered back-projection under reconstruct()'s schedule, and the learned
+initialiser) to the workload's "1e-4 relative cost change", and reports how
+far each is then from C*, the MAP cost of a long run cached in
+tests/golden/converged_deploy4d.npz; ``--until map`` runs on to
+|C - C*| <= 1e-4 |C*| with the object within 0.5% RMSE of x*.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -21,17 +31,16 @@ import os
 import sys
 import time
 
-import numpy as np
-
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "time-to-converged MAP (1e-4 rel) & voxel updates/s, learned vs plain init"
-
-
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
-                               "pmc_traffic_issue_k_icd_mw_interlaced4d_256x256x8x16.md")
+WORKLOAD = ("4D interlaced-view MBIR, deployment scan, finest stage: 512x512x32 slices x 16 "
+            "frames, 2048 views K=8, 512 bins, I0=1e4, zingers 0.1%, rings N(0,0.01)")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_icd_mwg_deploy4d_512x512x32x16.md")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "converged_deploy4d.npz")
+WEIGHTS = os.path.join(ROOT, "timbir_amd", "weights", "unet_init.npz")
 
 
 def measured_traffic():
@@ -54,110 +63,47 @@ def log(msg):
 def make_scan(small: bool):
     from timbir_amd import datagen
     if small:
-        return datagen.interlaced4d(N=64, nz=4, n_frames=4, n_views=256, K=8, ch_steps=16,
+        return datagen.interlaced4d(N=64, nz=8, n_frames=4, n_views=256, K=8, ch_steps=16,
                                     frame_steps=2)
-    return datagen.interlaced4d()
+    return datagen.deploy4d()
 
 
-def cpu_baseline(scan, stage_kw, threads: int, budget_s: float = 20.0):
-    """The oracle (CPU restatement, reference schedule) on the same stage:
-    voxel updates per second over whole ICD iterations, bounded in time."""
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+def cpu_threads() -> int:
+    """The CPUs this process may run on (the host cores the box gives it)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(scan, stage_kw, threads: int, budget_s: float = 15.0):
+    """The oracle (the CPU restatement, colour schedule) on the same finest
+    stage: HICD colour phases, one after the other, until budget_s of work;
+    voxel updates per second."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import pyorc
-    o = pyorc.Problem(scan, opt=pyorc.options(schedule=pyorc.SCHED_REFERENCE, num_threads=threads),
+    t0 = time.perf_counter()
+    o = pyorc.Problem(scan, opt=pyorc.options(schedule=pyorc.SCHED_COLOUR, num_threads=threads),
                       **stage_kw)
     o.init_error_sino()
+    setup = time.perf_counter() - t0
+    o.L.orc_set_select_counts(o.h, o.info["U"])
     t0 = time.perf_counter()
-    it = 0
-    while True:
-        it += 1
-        o.icd_iteration(it)
-        el = time.perf_counter() - t0
-        if el > budget_s or it >= 3:
+    phases = 0
+    for j in range(10):
+        for col in (0, 1):
+            o.run_phase(col, 1, j, 1)
+            phases += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        if time.perf_counter() - t0 > budget_s:
             break
+    el = time.perf_counter() - t0
     upd = o.update_count
     return dict(value=upd / el, unit="voxel updates/s", cores=threads, kind="port",
-                sample=f"{it} full ICD iteration(s) of the same stage, reference schedule, "
-                       f"{upd} voxel updates in {el:.1f} s")
-
-
-def time_to_converged(scan, finest_cap: int = 0, tol: float = 1e-4, init: str = "zero"):
-    """Plain initialisation (init "zero", the reference's, or "fbp": stage 0
-    from the filtered back-projection) under reconstruct()'s multires
-    schedule on the GPU, every stage stopped by the reference's own rule (convergence
-    ratio below convg_thresh after iteration 1, XT_ICD_update.c:1066-1104 and
-    :1188): `seconds` is the wall time from the start of the first stage to
-    that stop on the finest stage, with the MAP cost evaluated every finest
-    iteration.  With finest_cap > 0 the finest stage then runs on to
-    finest_cap iterations so that the cost trajectory shows how far that stop
-    is from the MAP point: `cost_gap_at_stop` = (C_stop - C_cap) / |C_cap|."""
-    import timbir_amd as tb
-    from timbir_amd.batch import multires_schedule
-    sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
-    t0 = time.perf_counter()
-    prev = None
-    var_est = 1.0
-    trace = []
-    stages = []
-    stop_t = stop_it = None
-    for s, (mxy, mz) in enumerate(sched):
-        ts = time.perf_counter()
-        st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
-                      prev_mult_z=sched[s - 1][1] if s else 0, var_est=var_est)
-        if prev is not None:
-            # the handoff of reconstruct(): upsampled from the previous stage in HBM
-            st.init_from_stage(prev)
-            st.set_offset(prev.offset())
-            prev.close()
-        elif init == "fbp":
-            st.init_fbp()
-        st.init_error_sino()
-        finest = s == len(sched) - 1
-        rule_it = None
-        cap = max(finest_cap, 1000 if finest_cap <= 0 else finest_cap) if finest else 1000
-        for it in range(1, cap + 1):
-            conv, _ = st.icd_iteration(it)
-            if finest:
-                c = st.cost()[0]
-                st.synchronize()
-                trace.append((time.perf_counter() - t0, c))
-            if conv and it > 1 and rule_it is None:
-                rule_it = it
-                if finest:
-                    stop_t, stop_it = trace[-1][0], it
-                    if finest_cap <= 0:
-                        break
-                else:
-                    break
-        st.synchronize()
-        ks = st.stats()
-        stages.append(dict(mult=[mxy, mz], rule_stop_iteration=rule_it,
-                           iterations_run=it, seconds=time.perf_counter() - ts,
-                           var_est=st.var_est, voxel_kernel_ms=ks["voxel_ms"],
-                           other_kernels_ms=ks["other_ms"], updates=ks["updates"],
-                           voxel_launches=ks["voxel_launches"],
-                           lds_launches=ks["lds_launches"],
-                           global_launches=ks["global_launches"]))
-        if not finest:
-            prev = st
-            var_est = st.var_est
-        else:
-            st.close()
-    marks = [i for i in (1, 2, 5, 10, 20, 50, 100, 200, 400, 800) if i <= len(trace)]
-    out = dict(seconds=stop_t, finest_rule_stop_iteration=stop_it, stages=stages,
-               finest_iterations_run=len(trace), cost_at_stop=trace[stop_it - 1][1] if stop_it
-               else None, finest_cost_at={str(i): trace[i - 1][1] for i in marks},
-               schedule=[list(x) for x in sched],
-               init="zero (plain)" if init == "zero" else "filtered back-projection at stage 0",
-               rule="convergence ratio < convg_thresh (reference rule)")
-    if finest_cap > 0:
-        ccap = trace[-1][1]
-        out["c_cap"] = ccap
-        out["cost_gap_at_stop"] = ((trace[stop_it - 1][1] - ccap) / abs(ccap)) if stop_it else None
-        k = next((i for i, (t, c) in enumerate(trace) if abs(c - ccap) <= tol * abs(ccap)), None)
-        out["seconds_to_within_tol_of_c_cap"] = trace[k][0] if k is not None else None
-        out["tol"] = tol
-    return out
+                sample=f"{phases} HICD colour phases (of 20 per ICD iteration) of the same "
+                       f"finest stage, {upd} voxel updates in {el:.1f} s, {threads} OpenMP "
+                       f"threads (set-up {setup:.1f} s not counted)")
 
 
 def copy_calibration(nbytes: int = 1 << 30, reps: int = 3):
@@ -180,24 +126,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--small", action="store_true", help="tiny workload for a quick check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="OpenMP threads of the CPU baseline (default: the CPUs this process "
+                         "may run on)")
     ap.add_argument("--no-converge", action="store_true",
-                    help="skip the time-to-converged run (plain init, reference stop rule)")
-    ap.add_argument("--converge", action="store_true",
-                    help="run the finest stage on past the stop rule to --finest-cap "
-                         "iterations, to show the remaining cost gap")
-    ap.add_argument("--finest-cap", type=int, default=400,
-                    help="finest-stage iterations run by --converge")
+                    help="skip the time-to-converged runs")
+    ap.add_argument("--starts", default="zero,fbp,learned",
+                    help="starts timed to convergence (zero, fbp, learned)")
+    ap.add_argument("--cap", type=int, default=60,
+                    help="finest-stage ICD iterations at most per start")
+    ap.add_argument("--until", default="rel_change", choices=["rel_change", "map"],
+                    help="rel_change: stop at the 1e-4 relative cost change; map: run on to "
+                         "|C - C*| <= 1e-4 |C*| and 0.5%% RMSE to x*")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (the scale for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
-    ap.add_argument("--deploy4d", action="store_true",
-                    help="also time the multires schedule to the stop rule on the deployment-"
-                         "size scan (512x512x32 x 16 frames, 2048 views; about a minute to "
-                         "generate)")
-    ap.add_argument("--precision3d", action="store_true",
-                    help="also time the multires schedule to the stop rule on the single-"
-                         "frame precision-study scan (512x512x32, 1440 views)")
+    ap.add_argument("--profile-steps", action="store_true",
+                    help="only the timed steps (for counter and trace runs)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -208,13 +153,13 @@ def main():
         sys.exit(2)
 
     import timbir_amd as tb
+    from timbir_amd.batch import multires_schedule
     tb.set_options(verbose=0, write_files=0)
     t0 = time.perf_counter()
     scan = make_scan(args.small)
     log(f"scan generated in {time.perf_counter() - t0:.1f} s")
-    # the finest stage of reconstruct()'s schedule for this scan: (1, 1) after
-    # (4, 4) and (2, 2)
-    stage_kw = dict(mult_xy=1, mult_z=1, stage_idx=2, prev_mult_z=2)
+    sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
+    stage_kw = dict(mult_xy=1, mult_z=1, stage_idx=len(sched) - 1, prev_mult_z=sched[-2][1])
     st = tb.Stage(scan, **stage_kw)
     info = st.info
     st.init_error_sino()
@@ -224,6 +169,7 @@ def main():
         st.icd_iteration(it)
     st.synchronize()
     st.reset_stats()
+    log(f"{args.steps} timed ICD iterations ...")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         it += 1
@@ -232,10 +178,12 @@ def main():
     el = time.perf_counter() - t0
     s = st.stats()
     cost = st.cost()[0]
+    st.close()
     value = s["updates"] / el
     vox_s = s["voxel_ms"] / 1e3
     achieved = s["alg_bytes"] / vox_s / 1e9 if vox_s > 0 else 0.0
     traffic = None if args.small else measured_traffic()
+    fam = s["family_launches"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -250,45 +198,57 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Cahn-Hilliard phantom, Poisson counts, zingers, rings)",
         "config": {
-            "workload": "4D interlaced-view MBIR, finest stage: 256x256x8 slices x 16 frames, "
-                        "1024 views K=8, 256 bins, I0=1e4, zingers 0.1%, rings N(0,0.01)"
-                        if not args.small else "small check workload",
+            "workload": WORKLOAD if not args.small else "small check workload",
             "N_x": info["N_x"], "N_z": info["N_z"], "N_time": info["N_time"],
-            "N_p": info["N_p"], "N_r": info["N_r"], "views_per_frame":
-                info["N_p"] // info["N_time"],
+            "N_p": info["N_p"], "N_r": info["N_r"],
+            "views_per_frame": info["N_p"] // info["N_time"],
         },
         "voxel_kernel": {"updates": s["updates"], "zero_skips": s["zero_skips"],
                          "launches": s["voxel_launches"], "time_ms": s["voxel_ms"],
                          "updates_per_s": s["updates"] / vox_s if vox_s > 0 else None,
-                         "entries": s["entries"]},
+                         "entries": s["entries"], "family_launches": fam},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_icd_mwg (the multi-wavefront voxel update, e in global "
+                               "memory)" if fam[tb.K_MWG] == s["voxel_launches"] else "mixed",
                      "alg_bytes_per_launch": s["alg_bytes"] / max(s["voxel_launches"], 1),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None},
         "cost_after": cost,
     }
-    if not args.no_converge and not args.small:
-        log("time to converged ...")
-        line["time_to_converged"] = time_to_converged(
-            scan, finest_cap=args.finest_cap if args.converge else 0)
-        log("time to converged from the filtered back-projection ...")
-        line["time_to_converged_fbp_init"] = time_to_converged(
-            scan, finest_cap=args.finest_cap if args.converge else 0, init="fbp")
-    if args.deploy4d:
-        from timbir_amd import datagen
-        log("deployment-size scan ...")
-        big = datagen.deploy4d()
-        log("time to converged, deployment size ...")
-        line["time_to_converged_deploy4d"] = time_to_converged(big)
-        del big
-    if args.precision3d:
-        from timbir_amd import datagen
-        log("time to converged, precision-study scan ...")
-        line["time_to_converged_precision3d"] = time_to_converged(datagen.precision3d())
-    if not args.no_cpu:
+    if args.profile_steps:
+        print(json.dumps(line), flush=True)
+        return
+    if not args.no_converge:
+        from timbir_amd import converge
+        golden = None if args.small else converge.load_golden(GOLDEN, scan)
+        net = None
+        starts = [x for x in args.starts.split(",") if x]
+        if "learned" in starts:
+            if os.path.exists(WEIGHTS) and not args.small:
+                from timbir_amd.learned import UNet
+                net = UNet(params=UNet.load_params(WEIGHTS))
+            else:
+                starts.remove("learned")
+        until = ("rel_change",) if args.until == "rel_change" else ("rel_change", "map")
+        ttc = {}
+        for start in starts:
+            log(f"time to converged from {start} ...")
+            ttc[start] = converge.run(scan, start, tol=1e-4, cap=args.cap, golden=golden,
+                                      net=net, until=until)
+        ttc["golden"] = ({"cost": golden["cost"], "iterations": golden["iterations"],
+                          "rel_stop": golden["rel_stop"], "file": os.path.relpath(GOLDEN, ROOT)}
+                         if golden else None)
+        if "learned" in ttc and "zero" in ttc:
+            sp = {}
+            for crit in ("rel_change", "map"):
+                a, b = ttc["zero"].get(crit), ttc["learned"].get(crit)
+                if a and b:
+                    sp[crit] = a["seconds"] / b["seconds"]
+            ttc["learned_speedup_vs_zero"] = sp
+        line["time_to_converged"] = ttc
+    if not args.no_cpu and not args.small:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads)
-    st.close()
+        line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads or cpu_threads())
     print(json.dumps(line), flush=True)
 
 

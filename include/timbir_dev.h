@@ -241,13 +241,23 @@ int tb_stage_object_device(tb_stage *st, void **ptr, uint64_t *bytes);
 
 /* ---- learned initialiser: kernels of a 3D U-Net (tb_unet.hip) ----------
  * No counterpart in the reference (its first stage starts from zero,
- * XT_ICD_update.c:549-553).  Pointers are device addresses; `stream` is a
- * hipStream_t (NULL: the default stream); every call is asynchronous on it.
+ * XT_ICD_update.c:549-553).  Pointers are device addresses (tb_dev_malloc);
+ * `stream` is a hipStream_t of this library's runtime (NULL: the default
+ * stream); every call is asynchronous on it.
  * Activations and activation gradients are bf16 [n][d][h][w][c] (channels
  * last, c a multiple of 32); convolution weights bf16 [27][cout][cin] with
  * tap = (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1); biases, weight gradients
  * and optimiser state fp32.  Return 0, or -1 with tb_nn_last_error(). */
 const char *tb_nn_last_error(void);
+/* Device memory from this library's HIP runtime, for the buffers of these
+ * kernels: allocate / free, blocking copies (kind 0 host to device, 1
+ * device to host, 2 device to device), zero fill on a stream, and a wait
+ * for all work on the device. */
+int tb_dev_malloc(uint64_t bytes, void **ptr);
+int tb_dev_free(void *ptr);
+int tb_dev_copy(void *dst, const void *src, uint64_t bytes, int32_t kind);
+int tb_dev_zero(void *ptr, uint64_t bytes, void *stream);
+int tb_dev_synchronize(void);
 /* 3x3x3 convolution with zero padding on the matrix cores (bf16 x bf16,
  * fp32 accumulate): out = [ReLU](bias + conv(in, w)); out fp32 when
  * out_f32, else bf16.  bias may be NULL. */

@@ -15,6 +15,15 @@ The same chain, held to the FP64 oracle's converged point, is
 tests/test_gpu_reconstruct.py::test_converges_to_the_fp64_map_point_at_full_size
 on the plain 3D workload.
 
+The finest stage does not reach REL_STOP in practice: past its first
+iterations it lowers the cost by a nearly constant ~4e-6 of its value per
+iteration for hundreds of iterations, a slow drift along the near-null
+direction that trades rotationally symmetric object components against
+the per-detector ring offsets (the projection of such a component is the
+same in every view, as an offset is).  The run therefore stops at CAP and
+its last cost is an upper bound on the MAP cost, saved every SAVE_EVERY
+iterations.
+
 Kept: C* (the last cost), the cost trace, and x* sampled on a fixed grid
 (every 4th slice, every 8th row and column: 16 x 8 x 64 x 64 values, float16,
 whose rounding is far below the 0.5%-of-range RMSE bar) with the scan's
@@ -37,9 +46,10 @@ from timbir_amd import datagen  # noqa: E402
 from timbir_amd.batch import multires_schedule  # noqa: E402
 
 REL_STOP = 1e-8
-CAP = 700
+CAP = 300
 COARSE_REL = 1e-6
 COARSE_CAP = 300
+SAVE_EVERY = 25
 SAMPLE = (slice(None), slice(None, None, 4), slice(None, None, 8), slice(None, None, 8))
 
 
@@ -51,6 +61,7 @@ def main(out):
     tb.set_options(verbose=0)
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     prev, var = None, 1.0
+    coarse = []
     for s, (mxy, mz) in enumerate(sched):
         st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
                       prev_mult_z=sched[s - 1][1] if s else 0, var_est=var)
@@ -68,9 +79,20 @@ def main(out):
                     break
             print(f"stage ({mxy},{mz}): {len(c) - 1} iterations, cost {c[-1]:.10e} "
                   f"({time.time() - t0:.0f} s)", flush=True)
+            coarse.append(len(c) - 1)
             prev, var = st, st.var_est
     st.init_error_sino()
     trace = [st.cost()[0]]
+
+    def save(it):
+        x = st.object()
+        np.savez_compressed(out, cost=np.float64(trace[-1]), trace=np.asarray(trace),
+                            x_sample=x[SAMPLE].astype(np.float16),
+                            x_range=np.float64(x.max() - x.min()),
+                            x_sum=np.float64(x.astype(np.float64).sum()), iterations=np.int32(it),
+                            rel_stop=np.float64(REL_STOP), var_est=np.float64(st.var_est),
+                            coarse_iterations=np.asarray(coarse, np.int32),
+                            scan_sha256=np.array(sha))
     it = 0
     for it in range(1, CAP + 1):
         st.icd_iteration(it)
@@ -79,14 +101,11 @@ def main(out):
         if it % 10 == 0:
             print(f"  it {it}: cost {trace[-1]:.12e} rel change {rel:.3e} "
                   f"({time.time() - t0:.0f} s)", flush=True)
+        if it % SAVE_EVERY == 0:
+            save(it)
         if rel < REL_STOP and it > 2:
             break
-    x = st.object()
-    np.savez_compressed(out, cost=np.float64(trace[-1]), trace=np.asarray(trace),
-                        x_sample=x[SAMPLE].astype(np.float16), x_range=np.float64(x.max() - x.min()),
-                        x_sum=np.float64(x.astype(np.float64).sum()), iterations=np.int32(it),
-                        rel_stop=np.float64(REL_STOP), var_est=np.float64(st.var_est),
-                        scan_sha256=np.array(sha))
+    save(it)
     st.close()
     print(f"finest: {it} iterations, C* = {trace[-1]:.12e} ({time.time() - t0:.0f} s)", flush=True)
 

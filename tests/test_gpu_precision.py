@@ -93,8 +93,10 @@ def test_precision_variants(opts):
         log(f"  {name:6s}: MAP cost vs f64 {out[name][0]:.2e} relative, rmse vs f64 "
             f"{out[name][1]:.2e} of range")
     log(f"  f64 MAP cost {c64:.10e}")
+    # the MAP cost bar of the path (1e-5 relative to the FP64 run), and the
+    # object well inside 0.5% of the range
     for name in ("fp32c", "fp32", "fp64c"):
-        assert out[name][0] < 1e-6 and out[name][1] < 2e-4, name
+        assert out[name][0] < 1e-5 and out[name][1] < 2e-4, name
     # the compensated FP32 sums are no worse than the FP64-theta1 default
     # beyond FP32 rounding of the A values, which all GPU variants share
     assert out["fp32c"][1] <= 1.5 * out["fp64c"][1] + 1e-6

@@ -14,9 +14,9 @@ import pytest
 
 import timbir_amd as tb
 from oracle import unet_ref as U
+from timbir_amd.devmem import DevArray, synchronize
 
 pytestmark = pytest.mark.gpu
-torch = pytest.importorskip("torch")
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -29,27 +29,27 @@ def log(msg):
 @pytest.fixture(scope="module")
 def nn():
     from timbir_amd import learned
-    assert torch.cuda.is_available()
     return learned
 
 
-def dev(a, dtype=None):
-    t = torch.as_tensor(np.ascontiguousarray(a))
-    if dtype is not None:
-        t = t.to(dtype)
-    return t.cuda()
+def dev(a):
+    return DevArray.from_numpy(np.ascontiguousarray(a, np.float32))
 
 
 def bf(a):
-    return dev(np.asarray(a, np.float32), torch.bfloat16)
+    return DevArray.from_numpy(np.asarray(a, np.float32), "bf16")
+
+
+def empty(shape, dtype="f32"):
+    return DevArray(shape, dtype)
 
 
 def host(t):
-    return t.float().cpu().numpy()
+    return t.numpy().astype(np.float32)
 
 
 def ptr(t):
-    return C.c_void_p(t.data_ptr()) if t is not None else None
+    return t.ptr if t is not None else None
 
 
 def rel(a, b):
@@ -70,39 +70,35 @@ def test_conv3d_matches_oracle(nn, shape, cin, cout):
     b = rng.standard_normal(cout).astype(np.float32)
     g = U.bf16(rng.standard_normal(shape + (cout,)).astype(np.float32))
     log(f"conv3d {shape} {cin}->{cout}")
-    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    s = None
     xt, wt, bt, gt = bf(x), bf(w), dev(b), bf(g)
-    y = torch.empty(shape + (cout,), dtype=torch.float32, device="cuda")
+    y = empty(shape + (cout,))
     assert L.tb_nn_conv3d(ptr(xt), ptr(wt), ptr(bt), ptr(y), *shape, cin, cout, 0, 1, s) == 0
     ref = U.conv3d(x.astype(np.float64), w.astype(np.float64), b)
-    torch.cuda.synchronize()
     err = np.abs(host(y) - ref).max() / np.abs(ref).max()
     log(f"  forward: max error {err:.2e} of max")
     assert err < 5e-5
     # ReLU and bf16 output
-    yb = torch.empty(shape + (cout,), dtype=torch.bfloat16, device="cuda")
+    yb = empty(shape + (cout,), "bf16")
     assert L.tb_nn_conv3d(ptr(xt), ptr(wt), ptr(bt), ptr(yb), *shape, cin, cout, 1, 0, s) == 0
-    torch.cuda.synchronize()
     got = host(yb)
     want = U.bf16(np.maximum(ref, 0).astype(np.float32))
     assert np.mean(got != want) < 2e-3 and np.abs(got - want).max() <= 1e-2 * np.abs(want).max()
     # input gradient: the forward kernel with the flipped, transposed weights
-    wf = torch.empty((27, cin, cout), dtype=torch.bfloat16, device="cuda")
+    wf = empty((27, cin, cout), "bf16")
     assert L.tb_nn_conv3d_flip(ptr(wt), ptr(wf), cin, cout, s) == 0
-    gx = torch.empty(shape + (cin,), dtype=torch.float32, device="cuda")
+    gx = empty(shape + (cin,))
     assert L.tb_nn_conv3d(ptr(gt), ptr(wf), None, ptr(gx), *shape, cout, cin, 0, 1, s) == 0
     ref_gx = U.conv3d_bwd_data(g.astype(np.float64), w.astype(np.float64))
-    torch.cuda.synchronize()
     err = np.abs(host(gx) - ref_gx).max() / np.abs(ref_gx).max()
     log(f"  input gradient: max error {err:.2e} of max")
     assert err < 5e-5
     # weight gradient, several slice counts (the partial sums add in order)
     for slices in (1, 7, 64):
-        gw = torch.empty((27, cout, cin), dtype=torch.float32, device="cuda")
-        part = torch.empty(slices * 27 * cin * cout, dtype=torch.float32, device="cuda")
+        gw = empty((27, cout, cin))
+        part = empty((slices * 27 * cin * cout,))
         assert L.tb_nn_conv3d_wgrad(ptr(xt), ptr(gt), ptr(gw), ptr(part), slices, *shape, cin,
                                     cout, s) == 0
-        torch.cuda.synchronize()
         ref_gw = U.conv3d_wgrad(x.astype(np.float64), g.astype(np.float64))
         err = np.abs(host(gw) - ref_gw).max() / np.abs(ref_gw).max()
         log(f"  weight gradient ({slices} slices): max error {err:.2e} of max")
@@ -111,22 +107,21 @@ def test_conv3d_matches_oracle(nn, shape, cin, cout):
 
 def test_pooling_upsampling_and_head_match_oracle(nn):
     L = nn.nn_lib()
-    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    s = None
     rng = np.random.default_rng(3)
     shape = (2, 4, 6, 10)
     x = U.bf16(rng.standard_normal(shape + (32,)).astype(np.float32))
     xt = bf(x)
-    p = torch.empty((2, 2, 3, 5, 32), dtype=torch.bfloat16, device="cuda")
+    p = empty((2, 2, 3, 5, 32), "bf16")
     assert L.tb_nn_avgpool2(ptr(xt), ptr(p), *shape, 32, 0, s) == 0
-    gp = torch.empty(shape + (32,), dtype=torch.bfloat16, device="cuda")
+    gp = empty(shape + (32,), "bf16")
     assert L.tb_nn_avgpool2(ptr(p), ptr(gp), *shape, 32, 1, s) == 0
     a = U.bf16(rng.standard_normal((2, 2, 3, 5, 64)).astype(np.float32))
-    cat = torch.empty(shape + (96,), dtype=torch.bfloat16, device="cuda")
+    cat = empty(shape + (96,), "bf16")
     assert L.tb_nn_up2_concat(ptr(bf(a)), ptr(xt), ptr(cat), *shape, 64, 32, s) == 0
-    ga = torch.empty((2, 2, 3, 5, 64), dtype=torch.bfloat16, device="cuda")
-    gb = torch.empty(shape + (32,), dtype=torch.bfloat16, device="cuda")
+    ga = empty((2, 2, 3, 5, 64), "bf16")
+    gb = empty(shape + (32,), "bf16")
     assert L.tb_nn_up2_concat_bwd(ptr(cat), ptr(ga), ptr(gb), *shape, 64, 32, s) == 0
-    torch.cuda.synchronize()
     # the mean of 8 bf16 values in fp32, rounded once: equal to the oracle's
     # up to one rounding step where the sums are formed in another order
     want = U.bf16(U.avgpool2(x.astype(np.float64)).astype(np.float32))
@@ -144,8 +139,7 @@ def run_network(nn, params, fbp, mean, target):
     out = net.forward(f, m, keep=True)
     loss, g = net.loss_and_grad(out, t)
     net.backward(g)
-    torch.cuda.synchronize()
-    grads = {k: host(v) for k, v in net.grad.items()}
+    grads = net.grads()
     return host(out), loss, grads
 
 

@@ -27,7 +27,8 @@ EXPORTS = [
     "tb_nhicd_list", "tb_icd_iteration", "tb_stage_run", "tb_get_stats", "tb_reset_stats",
     "tb_synchronize", "tb_stage_object_device",
     # learned initialiser kernels (tb_unet.hip)
-    "tb_nn_last_error", "tb_nn_conv3d", "tb_nn_conv3d_flip", "tb_nn_conv3d_wgrad",
+    "tb_nn_last_error", "tb_dev_malloc", "tb_dev_free", "tb_dev_copy", "tb_dev_zero",
+    "tb_dev_synchronize", "tb_nn_conv3d", "tb_nn_conv3d_flip", "tb_nn_conv3d_wgrad",
     "tb_nn_channel_sum", "tb_nn_relu_bwd", "tb_nn_avgpool2", "tb_nn_up2_concat",
     "tb_nn_up2_concat_bwd", "tb_nn_add", "tb_nn_make_input", "tb_nn_frame_mean", "tb_nn_head", "tb_nn_head_bwd",
     "tb_nn_mse", "tb_nn_sum_parts", "tb_nn_adam", "tb_nn_to_bf16",

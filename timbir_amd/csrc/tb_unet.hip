@@ -492,6 +492,42 @@ static int grid1d(size_t n) {
 
 extern "C" const char *tb_nn_last_error(void) { return g_nn_err.c_str(); }
 
+// Device memory for the callers of these kernels, from this library's HIP
+// runtime (so that the kernels and the copies are ordered on its streams).
+extern "C" int tb_dev_malloc(uint64_t bytes, void **ptr) {
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) return nn_fail("tb_dev_malloc", e);
+  return 0;
+}
+
+extern "C" int tb_dev_free(void *ptr) {
+  hipError_t e = hipFree(ptr);
+  if (e != hipSuccess) return nn_fail("tb_dev_free", e);
+  return 0;
+}
+
+extern "C" int tb_dev_copy(void *dst, const void *src, uint64_t bytes, int32_t kind) {
+  // kind: 0 host to device, 1 device to host, 2 device to device; blocking
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  hipError_t e = hipMemcpy(dst, src, bytes, k);
+  if (e != hipSuccess) return nn_fail("tb_dev_copy", e);
+  return 0;
+}
+
+extern "C" int tb_dev_zero(void *ptr, uint64_t bytes, void *stream) {
+  hipError_t e = hipMemsetAsync(ptr, 0, bytes, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return nn_fail("tb_dev_zero", e);
+  return 0;
+}
+
+extern "C" int tb_dev_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return nn_fail("tb_dev_synchronize", e);
+  return 0;
+}
+
 extern "C" int tb_nn_conv3d(const void *in, const void *w, const float *bias, void *out,
                             int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin,
                             int32_t cout, int32_t relu, int32_t out_f32, void *stream) {

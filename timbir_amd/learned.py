@@ -16,17 +16,18 @@ activations, fp32 accumulation on the matrix cores):
     b1, b2 (128) -> up + [e2] -> d2a, d2 (64) -> up + [e1] -> d1a, d1 (32)
     -> out = fbp + (head . d1 + b) / scale
 Every kernel (convolution forward, weight and input gradients, pooling,
-upsampling, loss, Adam) is the library's own (tb_nn_*, tb_unet.hip).  torch
-only allocates the device buffers and supplies the stream.
+upsampling, loss, Adam) is the library's own (tb_nn_*, tb_unet.hip); the
+buffers come from its own allocator (timbir_amd.devmem), and everything runs
+on its runtime's default stream.
 """
 from __future__ import annotations
 
 import ctypes as C
-import time
 
 import numpy as np
 
 from . import lib as _lib
+from .devmem import DevArray, synchronize
 
 # (name, input, Cin, Cout): 3x3x3 convolution + ReLU, in order
 ARCH = [
@@ -67,6 +68,7 @@ def nn_lib():
         "tb_nn_sum_parts": [vp, vp, i64, i32, vp],
         "tb_nn_adam": [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp],
         "tb_nn_to_bf16": [vp, vp, i64, vp],
+        "tb_nn_add": [vp, vp, i64, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -84,73 +86,72 @@ def _ok(rc):
 
 
 def _p(t):
-    return C.c_void_p(t.data_ptr())
+    return t.ptr if t is not None else None
+
+
+S0 = None   # the runtime's default stream
 
 
 class UNet:
     """The network's parameters and optimiser state on the GPU, and its
     forward and backward passes built from the tb_nn_* kernels."""
 
-    def __init__(self, params: dict | None = None, seed: int = 0, device: str = "cuda"):
-        import torch
-        self.torch = torch
-        self.dev = torch.device(device)
+    def __init__(self, params: dict | None = None, seed: int = 0):
         if params is None:
             params = init_params(seed)
         self.master, self.wbf, self.m, self.v, self.grad = {}, {}, {}, {}, {}
         for name, (w, b) in params.items():
             for k, arr in (("w", w), ("b", b)):
-                t = torch.as_tensor(np.ascontiguousarray(arr, np.float32)).to(self.dev)
+                t = DevArray.from_numpy(np.ascontiguousarray(arr, np.float32))
                 self.master[(name, k)] = t
-                self.m[(name, k)] = torch.zeros_like(t)
-                self.v[(name, k)] = torch.zeros_like(t)
-                self.grad[(name, k)] = torch.zeros_like(t)
+                self.m[(name, k)] = DevArray(t.shape, "f32", zero=True)
+                self.v[(name, k)] = DevArray(t.shape, "f32", zero=True)
+                self.grad[(name, k)] = DevArray(t.shape, "f32", zero=True)
             if name != "head":
-                self.wbf[name] = self.master[(name, "w")].to(torch.bfloat16)
+                wb = DevArray(self.master[(name, "w")].shape, "bf16")
+                _ok(nn_lib().tb_nn_to_bf16(_p(self.master[(name, "w")]), _p(wb), wb.numel(), S0))
+                self.wbf[name] = wb
         self.step_count = 0
         self.acts: dict = {}
 
     # -- helpers ---------------------------------------------------------------
-    def _stream(self):
-        return C.c_void_p(self.torch.cuda.current_stream(self.dev).cuda_stream)
-
-    def _bf(self, shape):
-        return self.torch.empty(shape, dtype=self.torch.bfloat16, device=self.dev)
-
     def params(self) -> dict:
         """Host copies of the master parameters, {name: (w, b)}."""
         out = {}
         for name, _, _, _ in ARCH + [("head", None, None, None)]:
-            out[name] = (self.master[(name, "w")].cpu().numpy(),
-                         self.master[(name, "b")].cpu().numpy())
+            out[name] = (self.master[(name, "w")].numpy(), self.master[(name, "b")].numpy())
         return out
+
+    def grads(self) -> dict:
+        """Host copies of the last backward pass's gradients, {(name, 'w'|'b'): array}."""
+        return {k: v.numpy() for k, v in self.grad.items()}
 
     # -- forward ---------------------------------------------------------------
     def forward(self, fbp, mean=None, keep: bool = False, out=None):
         """fbp: float32 [N][D][H][W] on the device (D, H, W divisible by 4),
-        a torch tensor or (device address, shape); mean: float32 [D][H][W]
-        (shared by the N samples) or [N][D][H][W], or None.  Returns the
-        float32 [N][D][H][W] output, written to `out` (a tensor or a device
-        address; it may be fbp's own memory) when given."""
-        L, s = nn_lib(), self._stream()
+        a DevArray or (device address, shape); mean: a float32 DevArray
+        [D][H][W] (shared by the N samples) or [N][D][H][W], or None.
+        Returns the float32 [N][D][H][W] output as a DevArray, written to
+        `out` (a DevArray or a device address; it may be fbp's own memory)
+        when given."""
+        L, s = nn_lib(), S0
         if isinstance(fbp, tuple):
             f_addr, (N, D, H, W) = fbp
         else:
-            f_addr, (N, D, H, W) = fbp.data_ptr(), tuple(fbp.shape)
+            f_addr, (N, D, H, W) = fbp.addr, tuple(fbp.shape)
         if D % 4 or H % 4 or W % 4:
             raise ValueError(f"volume {D}x{H}x{W}: every side must be divisible by 4")
         nvox = N * D * H * W
         per = D * H * W
         if mean is not None and mean.numel() == nvox:
             per = nvox
-        x = self._bf((N, D, H, W, 32))
-        _ok(L.tb_nn_make_input(C.c_void_p(f_addr), _p(mean) if mean is not None else None,
-                               _p(x), nvox, per, SCALE, s))
+        x = DevArray((N, D, H, W, 32), "bf16")
+        _ok(L.tb_nn_make_input(C.c_void_p(f_addr), _p(mean), _p(x), nvox, per, SCALE, s))
         acts = {"x": (x, (N, D, H, W))}
         for name, src, cin, cout in ARCH:
             if src.startswith("pool:"):
                 a, (n_, d_, h_, w_) = acts[src[5:]]
-                xin = self._bf((n_, d_ // 2, h_ // 2, w_ // 2, cin))
+                xin = DevArray((n_, d_ // 2, h_ // 2, w_ // 2, cin), "bf16")
                 _ok(L.tb_nn_avgpool2(_p(a), _p(xin), n_, d_, h_, w_, cin, 0, s))
                 shp = (n_, d_ // 2, h_ // 2, w_ // 2)
                 acts[src] = (xin, shp)
@@ -159,18 +160,22 @@ class UNet:
                 ta, _ = acts[a]
                 tb_, shp = acts[b]
                 ca, cb = ta.shape[-1], tb_.shape[-1]
-                xin = self._bf(shp + (ca + cb,))
+                xin = DevArray(shp + (ca + cb,), "bf16")
                 _ok(L.tb_nn_up2_concat(_p(ta), _p(tb_), _p(xin), *shp, ca, cb, s))
                 acts[src] = (xin, shp)
             else:
                 xin, shp = acts[src]
-            y = self._bf(shp + (cout,))
+            y = DevArray(shp + (cout,), "bf16")
             _ok(L.tb_nn_conv3d(_p(xin), _p(self.wbf[name]), _p(self.master[(name, "b")]),
                                _p(y), *shp, cin, cout, 1, 0, s))
             acts[name] = (y, shp)
+            if not keep:
+                # free what no later layer reads
+                for k in [k for k in acts if k not in _LIVE[name]]:
+                    del acts[k]
         if out is None:
-            out = self.torch.empty((N, D, H, W), dtype=self.torch.float32, device=self.dev)
-        o_addr = out if isinstance(out, int) else out.data_ptr()
+            out = DevArray((N, D, H, W), "f32")
+        o_addr = out if isinstance(out, int) else out.addr
         _ok(L.tb_nn_head(_p(acts["d1"][0]), _p(self.master[("head", "w")]),
                          _p(self.master[("head", "b")]), C.c_void_p(f_addr), C.c_void_p(o_addr),
                          nvox, 32, SCALE, s))
@@ -182,37 +187,37 @@ class UNet:
         tb_init_fbp) by the network's output, in HBM: the temporal mean of
         the back-projection, then the network over a few frames at a time,
         written in place.  Returns when the GPU has finished."""
-        torch = self.torch
         stage.synchronize()
         addr, _ = stage.object_device()
         i = stage.info
         F, per = i["N_time"], i["N_z"] * i["N_y"] * i["N_x"]
         shape = (i["N_z"], i["N_y"], i["N_x"])
-        mean = torch.empty(shape, dtype=torch.float32, device=self.dev)
-        _ok(nn_lib().tb_nn_frame_mean(C.c_void_p(addr), _p(mean), F, per, self._stream()))
+        mean = DevArray(shape, "f32")
+        _ok(nn_lib().tb_nn_frame_mean(C.c_void_p(addr), _p(mean), F, per, S0))
         for f0 in range(0, F, frames_per_pass):
             n = min(frames_per_pass, F - f0)
             a = addr + f0 * per * 4
             self.forward((a, (n,) + shape), mean, out=a)
-        torch.cuda.synchronize(self.dev)
+        synchronize()
 
     # -- backward --------------------------------------------------------------
     def backward(self, gout):
         """Gradients of sum(gout * forward(...)) into self.grad (the forward
         must have run with keep=True)."""
-        L, s = nn_lib(), self._stream()
+        L, s = nn_lib(), S0
         acts = self.acts
-        torch = self.torch
         h, shp = acts["d1"]
         nvox = int(np.prod(shp))
-        part = torch.empty((SLICES, 33), dtype=torch.float32, device=self.dev)
-        gh = self._bf(shp + (32,))
+        part = DevArray((SLICES, 33), "f32")
+        gh = DevArray(shp + (32,), "bf16")
         _ok(L.tb_nn_head_bwd(_p(h), _p(self.master[("head", "w")]), _p(gout), _p(gh), _p(part),
                              SLICES, nvox, 32, SCALE, s))
-        hb = torch.empty(33, dtype=torch.float32, device=self.dev)
+        hb = DevArray((33,), "f32")
         _ok(L.tb_nn_sum_parts(_p(part), _p(hb), 33, SLICES, s))
-        self.grad[("head", "w")].copy_(hb[:32])
-        self.grad[("head", "b")].copy_(hb[32:])
+        synchronize()
+        both = hb.numpy()
+        self.grad[("head", "w")].upload(both[:32])
+        self.grad[("head", "b")].upload(both[32:])
         g = {"d1": gh}
         wpart = None
         for name, src, cin, cout in reversed(ARCH):
@@ -223,29 +228,29 @@ class UNet:
             nv = int(np.prod(shp))
             need = SLICES * 27 * cin * cout
             if wpart is None or wpart.numel() < need:
-                wpart = torch.empty(need, dtype=torch.float32, device=self.dev)
+                wpart = DevArray((need,), "f32")
             _ok(L.tb_nn_conv3d_wgrad(_p(xin), _p(gy), _p(self.grad[(name, "w")]), _p(wpart),
                                      SLICES, *shp, cin, cout, s))
             _ok(L.tb_nn_channel_sum(_p(gy), _p(self.grad[(name, "b")]), _p(wpart), SLICES, nv,
                                     cout, s))
             if src == "x":
                 continue
-            wt = self._bf((27, cin, cout))
+            wt = DevArray((27, cin, cout), "bf16")
             _ok(L.tb_nn_conv3d_flip(_p(self.wbf[name]), _p(wt), cin, cout, s))
-            gx = self._bf(shp + (cin,))
+            gx = DevArray(shp + (cin,), "bf16")
             _ok(L.tb_nn_conv3d(_p(gy), _p(wt), None, _p(gx), *shp, cout, cin, 0, 0, s))
             if src.startswith("pool:"):
                 tgt = src[5:]
                 _, tshp = acts[tgt]
-                gt = self._bf(tshp + (cin,))
+                gt = DevArray(tshp + (cin,), "bf16")
                 _ok(L.tb_nn_avgpool2(_p(gx), _p(gt), *tshp, cin, 1, s))
                 self._accumulate(g, tgt, gt)
             elif src.startswith("cat:"):
                 a, b = src[4:].split(",")
                 ta, ashp = acts[a]
                 ca, cb = ta.shape[-1], cin - ta.shape[-1]
-                ga = self._bf(ashp + (ca,))
-                gb = self._bf(shp + (cb,))
+                ga = DevArray(ashp + (ca,), "bf16")
+                gb = DevArray(shp + (cb,), "bf16")
                 _ok(L.tb_nn_up2_concat_bwd(_p(gx), _p(ga), _p(gb), *shp, ca, cb, s))
                 self._accumulate(g, a, ga)
                 self._accumulate(g, b, gb)
@@ -254,30 +259,28 @@ class UNet:
 
     def _accumulate(self, g, key, t):
         if key in g:
-            _ok(nn_lib().tb_nn_add(_p(g[key]), _p(t), t.numel(), self._stream()))
+            _ok(nn_lib().tb_nn_add(_p(g[key]), _p(t), t.numel(), S0))
         else:
             g[key] = t
 
     # -- training --------------------------------------------------------------
     def adam(self, lr: float, b1=0.9, b2=0.999, eps=1e-8):
-        L, s = nn_lib(), self._stream()
+        L = nn_lib()
         self.step_count += 1
         for key, p in self.master.items():
             name, kind = key
             wb = self.wbf[name] if (kind == "w" and name != "head") else None
             _ok(L.tb_nn_adam(_p(p), _p(self.grad[key]), _p(self.m[key]), _p(self.v[key]),
-                             _p(wb) if wb is not None else None, p.numel(), lr, b1, b2, eps,
-                             self.step_count, s))
+                             _p(wb), p.numel(), lr, b1, b2, eps, self.step_count, S0))
 
     def loss_and_grad(self, out, target):
-        """Mean squared error and its gradient (fp32)."""
-        L, s = nn_lib(), self._stream()
-        torch = self.torch
-        g = torch.empty_like(out)
+        """Mean squared error (a float, after a wait) and its gradient."""
+        L = nn_lib()
+        g = DevArray(out.shape, "f32")
         blocks = 256
-        part = torch.empty(blocks, dtype=torch.float64, device=self.dev)
-        _ok(L.tb_nn_mse(_p(out), _p(target), _p(g), _p(part), blocks, out.numel(), s))
-        return part.sum().item() / out.numel(), g
+        part = DevArray((blocks,), "f64")
+        _ok(L.tb_nn_mse(_p(out), _p(target), _p(g), _p(part), blocks, out.numel(), S0))
+        return float(part.numpy().sum()) / out.numel(), g
 
     def train_step(self, fbp, mean, target, lr):
         out = self.forward(fbp, mean, keep=True)
@@ -302,8 +305,31 @@ class UNet:
         z = np.load(path)
         out = {}
         for name, _, _, _ in ARCH + [("head", None, None, None)]:
-            out[name] = (z[name + ".w"], z[name + ".b"])
+            out[name] = (z[name + ".w"].astype(np.float32), z[name + ".b"].astype(np.float32))
         return out
+
+
+def _live_sets() -> dict:
+    """For each layer, the activations some later layer (or the head) still
+    reads once it has run: what an inference pass keeps."""
+    order = [a[0] for a in ARCH]
+    need = {}
+    for i, name in enumerate(order):
+        later = set()
+        for name2, src, _, _ in ARCH[i + 1:]:
+            if src.startswith("pool:"):
+                later.add(src[5:])
+            elif src.startswith("cat:"):
+                later.update(src[4:].split(","))
+            else:
+                later.add(src)
+        later.add(name)
+        later.add("d1")
+        need[name] = later
+    return need
+
+
+_LIVE = _live_sets()
 
 
 def init_params(seed: int = 0) -> dict:

@@ -32,6 +32,7 @@ import numpy as np
 
 from . import Stage, datagen, set_options
 from .batch import multires_schedule
+from .devmem import DevArray
 from .learned import UNet
 
 N, NZ, FRAMES, VIEWS, K, I0 = 256, 8, 16, 2048, 8, 1e4
@@ -139,24 +140,29 @@ def crops(data, rng, batch: int, crop: int):
         y0, x0 = rng.integers(0, N - crop + 1, 2)
         sl = (slice(None), slice(y0, y0 + crop), slice(x0, x0 + crop))
         fb.append(fbp[f][sl])
-        mn.append(fbp.mean(axis=0)[sl])
+        mn.append(fbp.mean(axis=0)[sl].astype(np.float32))
         tg.append(target[f][sl])
     return np.stack(fb), np.stack(mn), np.stack(tg)
 
 
 def train(net: UNet, data, steps: int, batch: int = 4, crop: int = 128, lr: float = 1e-3,
-          seed: int = 0, log_every: int = 100) -> list:
+          seed: int = 0, log_every: int = 100, budget_s: float = 1e30, checkpoint=None) -> list:
     """Adam on the squared error between the network's output and the MBIR
-    target over batches of random crops, cosine learning-rate decay."""
-    import torch
+    target over batches of random crops, cosine learning-rate decay; stops
+    early after budget_s seconds; checkpoint(step) every 500 steps."""
     rng = np.random.default_rng(seed)
     hist = []
     t0 = time.time()
     for step in range(1, steps + 1):
+        if time.time() - t0 > budget_s:
+            log(f"  training stopped by its time budget after {step - 1} steps")
+            break
+        if checkpoint and step % 500 == 0:
+            checkpoint(step)
         fb, mn, tg = crops(data, rng, batch, crop)
         cur = lr * 0.5 * (1 + np.cos(np.pi * (step - 1) / steps))
-        loss = net.train_step(torch.from_numpy(fb).cuda(), torch.from_numpy(mn).cuda(),
-                              torch.from_numpy(tg).cuda(), cur)
+        loss = net.train_step(DevArray.from_numpy(fb), DevArray.from_numpy(mn),
+                              DevArray.from_numpy(tg), cur)
         hist.append(loss)
         if step % log_every == 0 or step == 1:
             log(f"  step {step}: loss {np.mean(hist[-log_every:]):.4e} lr {cur:.2e} "
@@ -167,12 +173,11 @@ def train(net: UNet, data, steps: int, batch: int = 4, crop: int = 128, lr: floa
 def evaluate(net: UNet, val) -> dict:
     """Mean squared error to the target of the back-projection and of the
     network's output over the validation scans' full frames."""
-    import torch
     e_fbp, e_net = [], []
     for fbp, target in val:
-        f = torch.from_numpy(fbp).cuda()
-        m = torch.from_numpy(np.ascontiguousarray(fbp.mean(axis=0))).cuda()
-        out = net.forward(f, m).cpu().numpy()
+        f = DevArray.from_numpy(np.ascontiguousarray(fbp))
+        m = DevArray.from_numpy(np.ascontiguousarray(fbp.mean(axis=0), np.float32))
+        out = net.forward(f, m).numpy()
         e_fbp.append(float(np.mean((fbp - target) ** 2)))
         e_net.append(float(np.mean((out - target) ** 2)))
     return dict(mse_fbp=float(np.mean(e_fbp)), mse_net=float(np.mean(e_net)))
@@ -191,11 +196,18 @@ def main(argv=None):
     ap.add_argument("--out", default=os.path.join(os.path.dirname(__file__), "weights",
                                                   "unet_init.npz"))
     ap.add_argument("--report", default=None)
+    ap.add_argument("--data-budget", type=float, default=1e30,
+                    help="seconds of scan generation and reconstruction at most")
+    ap.add_argument("--train-budget", type=float, default=1e30,
+                    help="seconds of training at most")
     a = ap.parse_args(argv)
     set_options(verbose=0)
     t0 = time.time()
     data = []
     for k in range(a.scans + a.val):
+        if time.time() - t0 > a.data_budget and len(data) > a.val + 1:
+            log(f"data budget reached after {len(data)} scans")
+            break
         ts = time.time()
         scan = synth_scan(a.seed + k)
         tg = time.time() - ts
@@ -203,19 +215,21 @@ def main(argv=None):
         data.append((fbp, target))
         log(f"scan {k}: generated in {tg:.1f} s, reconstructed in {time.time() - ts - tg:.1f} s "
             f"(mse fbp {np.mean((fbp - target) ** 2):.3e})")
-    train_set, val = data[:a.scans], data[a.scans:]
+    val, train_set = data[:a.val], data[a.val:]
     t1 = time.time()
     net = UNet(seed=a.seed)
     before = evaluate(net, val)
-    hist = train(net, train_set, a.steps, a.batch, a.crop, a.lr, seed=a.seed)
+    os.makedirs(os.path.dirname(a.out) or ".", exist_ok=True)
+    hist = train(net, train_set, a.steps, a.batch, a.crop, a.lr, seed=a.seed,
+                 budget_s=a.train_budget,
+                 checkpoint=lambda st: net.save(a.out, scans=len(train_set), steps=st))
     after = evaluate(net, val)
     t2 = time.time()
     log(f"validation mse: back-projection {before['mse_fbp']:.4e}, untrained {before['mse_net']:.4e}, "
         f"trained {after['mse_net']:.4e}")
-    os.makedirs(os.path.dirname(a.out) or ".", exist_ok=True)
-    net.save(a.out, scans=a.scans, finest_iters=a.finest_iters, steps=a.steps, seed=a.seed,
+    net.save(a.out, scans=len(train_set), finest_iters=a.finest_iters, steps=a.steps, seed=a.seed,
              val_mse_fbp=after["mse_fbp"], val_mse_net=after["mse_net"])
-    rep = dict(scans=a.scans, val=a.val, finest_iters=a.finest_iters, steps=a.steps,
+    rep = dict(scans=len(train_set), val=a.val, steps_run=len(hist), finest_iters=a.finest_iters, steps=a.steps,
                batch=a.batch, crop=a.crop, lr=a.lr, seed=a.seed, data_s=t1 - t0,
                train_s=t2 - t1, before=before, after=after, loss=hist[::max(1, len(hist) // 100)])
     if a.report:
