@@ -14,12 +14,12 @@ updates per second over the timed steps (wall clock, all kernels included);
 the roofline block covers the voxel-update kernel alone (HIP events on the
 library's stream).
 
-``time_to_converged`` times the three starts of timbir_amd.converge (zero and
-filtered back-projection under reconstruct()'s schedule, and the learned
-initialiser) to the workload's "1e-4 relative cost change", and reports how
-far each is then from C*, the MAP cost of a long run cached in
-tests/golden/converged_deploy4d.npz; ``--until map`` runs on to
-|C - C*| <= 1e-4 |C*| with the object within 0.5% RMSE of x*.
+``time_to_converged`` times starts of timbir_amd.converge (by default zero
+under reconstruct()'s schedule, the filtered back-projection at the finest
+stage, and the learned initialiser) to the workload's "1e-4 relative cost
+change", and reports how far each is then from C*, the lowest MAP cost of a
+long run cached in tests/golden/converged_deploy4d.npz; ``--until map``
+runs on to |C - C*| <= 1e-4 |C*| with the object within 0.5% RMSE of x*.
 
 Prints one JSON line.
 """
@@ -60,12 +60,21 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_scan(small: bool):
+WORKLOADS = {
+    "deploy4d": WORKLOAD,
+    "interlaced4d": "4D interlaced-view MBIR, finest stage: 256x256x8 slices x 16 frames, 1024 "
+                    "views K=8, 256 bins, I0=1e4, zingers 0.1%, rings N(0,0.01)",
+    "precision3d": "3D MBIR, finest stage: 512x512x32 slices, 1440 views, 512 bins, I0=2e4, "
+                   "rings N(0,0.005)",
+}
+
+
+def make_scan(small: bool, workload: str = "deploy4d"):
     from timbir_amd import datagen
     if small:
         return datagen.interlaced4d(N=64, nz=8, n_frames=4, n_views=256, K=8, ch_steps=16,
                                     frame_steps=2)
-    return datagen.deploy4d()
+    return getattr(datagen, workload)()
 
 
 def cpu_threads() -> int:
@@ -125,14 +134,18 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--small", action="store_true", help="tiny workload for a quick check")
+    ap.add_argument("--workload", default="deploy4d", choices=sorted(WORKLOADS),
+                    help="the scan (default: the deployment scan the metric is quoted on; the "
+                         "others are for comparison, not bench lines)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="OpenMP threads of the CPU baseline (default: the CPUs this process "
                          "may run on)")
     ap.add_argument("--no-converge", action="store_true",
                     help="skip the time-to-converged runs")
-    ap.add_argument("--starts", default="zero,fbp,learned",
-                    help="starts timed to convergence (zero, fbp, learned)")
+    ap.add_argument("--starts", default="zero,fbp_finest,learned",
+                    help="starts timed to convergence (zero, fbp: back-projection at stage 0, "
+                         "fbp_finest: back-projection at the finest stage, learned)")
     ap.add_argument("--cap", type=int, default=60,
                     help="finest-stage ICD iterations at most per start")
     ap.add_argument("--until", default="rel_change", choices=["rel_change", "map"],
@@ -156,7 +169,7 @@ def main():
     from timbir_amd.batch import multires_schedule
     tb.set_options(verbose=0, write_files=0)
     t0 = time.perf_counter()
-    scan = make_scan(args.small)
+    scan = make_scan(args.small, args.workload)
     log(f"scan generated in {time.perf_counter() - t0:.1f} s")
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     stage_kw = dict(mult_xy=1, mult_z=1, stage_idx=len(sched) - 1, prev_mult_z=sched[-2][1])
@@ -198,7 +211,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Cahn-Hilliard phantom, Poisson counts, zingers, rings)",
         "config": {
-            "workload": WORKLOAD if not args.small else "small check workload",
+            "workload": WORKLOADS[args.workload] if not args.small else "small check workload",
             "N_x": info["N_x"], "N_z": info["N_z"], "N_time": info["N_time"],
             "N_p": info["N_p"], "N_r": info["N_r"],
             "views_per_frame": info["N_p"] // info["N_time"],
@@ -220,11 +233,12 @@ def main():
         return
     if not args.no_converge:
         from timbir_amd import converge
-        golden = None if args.small else converge.load_golden(GOLDEN, scan)
+        golden = (converge.load_golden(GOLDEN, scan)
+                  if args.workload == "deploy4d" and not args.small else None)
         net = None
         starts = [x for x in args.starts.split(",") if x]
         if "learned" in starts:
-            if os.path.exists(WEIGHTS) and not args.small:
+            if os.path.exists(WEIGHTS) and not args.small and args.workload != "precision3d":
                 from timbir_amd.learned import UNet
                 net = UNet(params=UNet.load_params(WEIGHTS))
             else:

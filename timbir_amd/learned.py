@@ -302,10 +302,15 @@ class UNet:
 
     @staticmethod
     def load_params(path: str) -> dict:
+        """Parameters saved by save(), or with the convolution weights as
+        bf16 bit patterns (uint16), the values the kernels use."""
+        from .devmem import from_bf16_bits
         z = np.load(path)
         out = {}
         for name, _, _, _ in ARCH + [("head", None, None, None)]:
-            out[name] = (z[name + ".w"].astype(np.float32), z[name + ".b"].astype(np.float32))
+            w = z[name + ".w"]
+            w = from_bf16_bits(w) if w.dtype == np.uint16 else w.astype(np.float32)
+            out[name] = (np.ascontiguousarray(w), z[name + ".b"].astype(np.float32))
         return out
 
 
