@@ -156,6 +156,10 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--profile-steps", action="store_true",
                     help="only the timed steps (for counter and trace runs)")
+    ap.add_argument("--kernel", type=int, default=0,
+                    help="voxel-update kernel (tb_options.kernel; 0: the fastest that applies)")
+    ap.add_argument("--row-lookahead", type=int, default=0,
+                    help="voxels' (e, w) rows in flight in the deep-pipeline kernel (0: default)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -167,7 +171,7 @@ def main():
 
     import timbir_amd as tb
     from timbir_amd.batch import multires_schedule
-    tb.set_options(verbose=0, write_files=0)
+    tb.set_options(verbose=0, write_files=0, kernel=args.kernel, row_lookahead=args.row_lookahead)
     t0 = time.perf_counter()
     scan = make_scan(args.small, args.workload)
     log(f"scan generated in {time.perf_counter() - t0:.1f} s")
@@ -222,8 +226,10 @@ def main():
                          "entries": s["entries"], "family_launches": fam},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_icd_mwg (the multi-wavefront voxel update, e in global "
-                               "memory)" if fam[tb.K_MWG] == s["voxel_launches"] else "mixed",
+                     "kernel": ("k_icd_mwd (the deep-pipeline multi-wavefront voxel update, e in "
+                                "global memory)" if fam[tb.K_MWD] == s["voxel_launches"] else
+                                "k_icd_mwg (the multi-wavefront voxel update, e in global memory)"
+                                if fam[tb.K_MWG] == s["voxel_launches"] else "mixed"),
                      "alg_bytes_per_launch": s["alg_bytes"] / max(s["voxel_launches"], 1),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None},
         "cost_after": cost,

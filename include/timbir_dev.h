@@ -59,7 +59,10 @@ typedef struct {
                                      6 = the pass kernel that holds its
                                      passes in registers alone, 7 = the
                                      pass kernel that walks any number of
-                                     passes in groups alone */
+                                     passes in groups alone, 8 = the
+                                     deep-pipeline multi-wavefront kernel
+                                     with e in global memory where it
+                                     applies (3- and 5-bin columns) */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront
@@ -81,6 +84,10 @@ typedef struct {
                                      global-e kernel for 3- and 5-bin
                                      columns (the two finest stages); a
                                      stage it does not serve fails */
+  int32_t row_lookahead;          /* voxels whose (e, w) rows are in
+                                     flight while one is solved, in the
+                                     deep-pipeline kernel: 0 = 3
+                                     (default), 1, 3 or 7 */
 } tb_options;
 
 void tb_get_options(tb_options *opt);
@@ -210,6 +217,8 @@ enum {
   TB_K_WIDE = 4,      /* k_icd_wide: columns of up to 17 bins */
   TB_K_WPASS = 5,     /* k_icd_wpass: views in passes held in registers */
   TB_K_WLOOP = 6,     /* k_icd_wloop: views in groups of passes */
+  TB_K_MWD = 7,       /* k_icd_mwd: as k_icd_mwg, rows of the next voxels
+                         in flight */
   TB_KERNEL_FAMILIES = 8
 };
 typedef struct {

@@ -41,7 +41,7 @@ class TbOptions(C.Structure):
                 ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32),
                 ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
                 ("init_fbp", C.c_int32), ("no_column_table", C.c_int32),
-                ("theta_fp", C.c_int32)]
+                ("theta_fp", C.c_int32), ("row_lookahead", C.c_int32)]
 
 
 class TbStats(C.Structure):
@@ -56,7 +56,7 @@ class TbStats(C.Structure):
 
 
 # voxel-update kernel families (TB_K_* in timbir_dev.h)
-K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP = range(7)
+K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP, K_MWD = range(8)
 
 
 class TimbirError(RuntimeError):

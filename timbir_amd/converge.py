@@ -21,7 +21,10 @@ Criteria, checked after every ICD iteration of the finest stage:
   map         |C_k - C*| <= tol |C*| and the RMSE between the object and x*
               at most rmse_frac of x*'s range, with C* and x* from a long
               run cached with the scan's SHA-256 (x* sampled on a fixed
-              grid).
+              grid);
+  target      the cost has come down to a given level C_k <= target (to
+              compare starts at one cost, e.g. where another start met
+              rel_change).
 
 The clock runs from the creation of the first stage to the iteration that
 meets a criterion, with the time spent evaluating the criteria (the cost,
@@ -57,10 +60,10 @@ class Clock:
         return ref - self.t0 - self.paused
 
 
-def _finest_loop(st, clock, cap, tol, golden, rmse_frac, trace, until):
+def _finest_loop(st, clock, cap, tol, golden, rmse_frac, trace, until, target):
     """ICD iterations of the finest stage until the criteria in `until`
-    ('rel_change', 'map') are met or cap iterations; fills trace."""
-    res = {"rel_change": None, "map": None}
+    ('rel_change', 'map', 'target') are met or cap iterations; fills trace."""
+    res = {"rel_change": None, "map": None, "target": None}
     st.synchronize()
     clock.pause()
     last = st.cost()[0]
@@ -84,6 +87,8 @@ def _finest_loop(st, clock, cap, tol, golden, rmse_frac, trace, until):
             rmse = float(np.sqrt(np.mean((x - golden["x"]) ** 2)) / golden["range"])
             if rmse <= rmse_frac:
                 res["map"] = dict(seconds=t, iterations=it, cost=c, gap=gap, rmse=rmse)
+        if target is not None and res["target"] is None and c <= target:
+            res["target"] = dict(seconds=t, iterations=it, cost=c, gap=gap)
         last = c
         clock.resume()
         if all(res[k] is not None for k in until):
@@ -93,10 +98,13 @@ def _finest_loop(st, clock, cap, tol, golden, rmse_frac, trace, until):
 
 def run(scan: dict, start: str = "zero", tol: float = 1e-4, cap: int = 100,
         golden: dict | None = None, rmse_frac: float = 5e-3, net=None,
-        until=("rel_change",)) -> dict:
+        until=("rel_change",), target: float | None = None) -> dict:
     """Time to the criteria from one start (see the module docstring).
     golden: {'cost', 'x' (sampled x*), 'sample' (index of the sample),
-    'range'} or None.  net: a learned.UNet for start 'learned'."""
+    'range'} or None.  net: a learned.UNet for start 'learned'.  target: the
+    cost level of the 'target' criterion."""
+    if "target" in until and target is None:
+        raise ValueError("criterion 'target' needs a target cost")
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     clock = Clock()
     stages = []
@@ -137,12 +145,13 @@ def run(scan: dict, start: str = "zero", tol: float = 1e-4, cap: int = 100,
                 prev, var = st, st.var_est
             else:
                 st.init_error_sino()
-    res, its = _finest_loop(st, clock, cap, tol, golden, rmse_frac, trace, until)
+    res, its = _finest_loop(st, clock, cap, tol, golden, rmse_frac, trace, until, target)
     stats = st.stats()
     st.close()
     marks = [i for i in (0, 1, 2, 5, 10, 20, 50, 100, 200, 400) if i < len(trace)]
     return dict(start=start, tol=tol, cap=cap, finest_iterations_run=its,
                 seconds_run=trace[-1][0], rel_change=res["rel_change"], map=res["map"],
+                target=res["target"], final_cost=trace[-1][1],
                 stages=stages, finest_cost_at={str(i): trace[i][1] for i in marks},
                 finest_gap_at={str(i): trace[i][2] for i in marks},
                 voxel_updates=stats["updates"], voxel_kernel_ms=stats["voxel_ms"])

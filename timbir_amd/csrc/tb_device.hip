@@ -32,6 +32,8 @@ template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mw(KP, const int2 *, int, int, int, int, int);
 template <int NB, int MZ, int VPL, int MAXTH, int TF>
 __global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
+template <int NB, int MZ, int VPL, int MAXTH, int D, int C>
+__global__ void k_icd_mwd(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int MAXTH>
 __global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int PMAX, int MAXTH, bool HL>
@@ -65,7 +67,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -973,6 +975,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
   else if (2 * st->mult_xy + 1 <= 5 && st->mult_z == 2) nb = 5;
   if (st->N_r < nb) nb = 0;
   bool launched = false;
+  bool deep = false;  // the launch below is k_icd_mwd
   auto run_mw = [&](mw_fn fn, int colnb, int nth, int vpw, size_t lds, bool global) -> int {
     // the column table may be allocated now: take the parameters again
     const int ec = ensure_columns(st, colnb);
@@ -988,7 +991,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
                        chunk, seq, count_mag, vpw);
     if (global) st->global_launches++;
     else st->lds_launches++;
-    st->fam_launches[global ? TB_K_MWG : TB_K_MW]++;
+    st->fam_launches[global ? (deep ? TB_K_MWD : TB_K_MWG) : TB_K_MW]++;
     launched = true;
     return 0;
   };
@@ -1019,6 +1022,35 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     }
     const size_t lds = icd_mw_lds_bytes(nv, st->N_r, st->mult_z, nth * vpl * 64);
     if (fn && lds + 1024 <= st->lds_max && run_mw(fn, nb, nth, vpw, lds, false) < 0) return -1;
+  }
+  if (!launched && tf == 0 && g_opt.kernel == 8) {
+    // the deep-pipeline global-e kernel: the lane layout of k_icd_mwg with
+    // `la` voxels' (e, w) rows in flight (tb_options.row_lookahead)
+    const int w = 2 * st->mult_xy + 1;
+    const int gnb = w <= 3 ? 3 : w <= 5 ? 5 : 0;
+    const int mz = st->mult_z;
+    const int la = g_opt.row_lookahead == 0 ? 3 : g_opt.row_lookahead;
+    if (la != 1 && la != 3 && la != 7) return fail("row_lookahead %d: 0, 1, 3 or 7", la);
+    auto pick = [&](int vpl, int nth) -> mw_fn {
+      if (gnb == 3 && mz == 1) {
+        if (la == 1) return vpl == 1 && nth <= 8 ? k_icd_mwd<3, 1, 1, 8, 1, 2> : nullptr;
+        if (la == 7) return vpl == 1 && nth <= 8 ? k_icd_mwd<3, 1, 1, 8, 7, 1> : nullptr;
+        if (vpl == 1) return nth <= 8 ? k_icd_mwd<3, 1, 1, 8, 3, 4> : k_icd_mwd<3, 1, 1, 15, 3, 4>;
+      } else if (gnb == 5 && mz == 2 && la == 3) {
+        if (vpl == 1) return nth <= 8 ? k_icd_mwd<5, 2, 1, 8, 3, 4> : k_icd_mwd<5, 2, 1, 15, 3, 4>;
+      }
+      return nullptr;
+    };
+    mw_fn fn = nullptr;
+    int nth = 0, vpw = 0;
+    if (gnb && st->N_r >= gnb) {
+      const int cap = 64 / gnb;  // one view per lane
+      nth = (st->max_nv + cap - 1) / cap;
+      vpw = (st->max_nv + nth - 1) / nth;
+      if (nth <= 15) fn = pick(1, nth);
+    }
+    deep = fn != nullptr;
+    if (fn && run_mw(fn, gnb, nth, vpw, 0, true) < 0) return -1;
   }
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 4)) {
     // lane = (view, bin) with the slice's mult_z rows: NB bins, VW = 64 / NB

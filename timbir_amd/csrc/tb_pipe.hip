@@ -551,7 +551,10 @@ template __global__ void k_build_columns<17>(KP);
 // the qGGMRF prior surrogate of XT_ICD_update.c:134-183, zero-skip,
 // XT_VoxUpdate.c:236-259, the object and update-map stores), meeting the
 // theta waves at one barrier per voxel.
-template <int MAXTH>
+// PAD: the theta waves run whole groups of G voxels, the last one padded
+// with empty slots; the prior wave meets them at every slot, publishing a
+// skipped step (dx = 0) for an empty one.
+template <int MAXTH, bool PAD = false>
 __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *list, int n0,
                                            int cnt, int f, int z, int iter, int seq,
                                            int count_mag, int nth, int lane) {
@@ -626,9 +629,20 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
 #pragma unroll
     for (int s = 0; s < G; s++) {
       const int pix = apix[s];
-      if (pix == kNoPix) continue;  // only past the end of the sequence
       const int n = (g - 1) * G + s;
       const int par = n & 1;
+      if (pix == kNoPix) {  // only past the end of the sequence
+        if (PAD && g > 0) {
+          if (lane == 0) {
+            S.pr[par][0] = 1.0f;
+            S.pr[par][1] = 0.0f;
+            S.pr[par][2] = 0.0f;
+            S.pr[par][3] = 1.0f;
+          }
+          __syncthreads();
+        }
+        continue;
+      }
       const float nbv = anb[s];
       const bool vld = (nbv == nbv) & (lane != 13);
       const float V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
@@ -1281,6 +1295,302 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *
 }
 
 // ---------------------------------------------------------------------------
+// k_icd_mwd: the global-e multi-wavefront update (k_icd_mwg's lane layout,
+// prior wave and exchange, theta1 in FP64) with a deeper pipeline, for
+// cells whose (e, w) rows miss the L2 and come from the memory-side cache
+// or HBM.  At voxel n a lane has the (e, w) rows of voxels n + 1 .. n + D
+// in flight and the column-table entries of voxels n + D + 1 .. n + D + C:
+//   issues the column entry of voxel n + D + C and the (e, w) rows of voxel
+//   n + D (addressed by the column entry that arrived C voxels ago),
+//   replays on voxel n's entries the updates of voxels n - D .. n - 1 that
+//   its rows were read before (the owner lanes' own fmaf(-a, dx, e), in
+//   the same order, so the bits are those of the sequential sweep),
+//   reduces voxel n's theta sums, takes voxel n's A value at the bins it
+//   holds for voxels n + 1 .. n + D (their replay coefficients), meets the
+//   other waves, solves and stores voxel n's e.
+// Only the waits of the compiler's own counters remain in the loop, so a
+// row waits for its data D voxels after it was issued.  D + 1 and C divide
+// the group of G voxels, so the unrolled group rotates the register rings
+// into place without copies of rows still in flight.
+// ---------------------------------------------------------------------------
+template <int NB, int MZ, int VPL, int MAXTH, int D, int C>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwd(KP P, const int2 *__restrict__ cells,
+                                                            int iter, int chunk, int seq,
+                                                            int count_mag, int vpw) {
+  constexpr int G = 8;  // pixels read per list load
+  static_assert(D >= 1 && C >= 1 && D + C <= G && G % (D + 1) == 0 && G % C == 0,
+                "look-ahead rings must rotate within one group");
+  constexpr int VW = 64 / NB;  // views of one lane layer
+  constexpr int REC = col_rec(NB);
+  __shared__ StepIn S;
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int Nxy = P.N_x * P.N_y;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0) return;
+  if (threadIdx.x < 32) {
+    (&S.t1[0][0])[threadIdx.x] = 0.0;
+    (&S.t2[0][0])[threadIdx.x] = 0.0f;
+  }
+  __syncthreads();
+
+  if (wave < nth) {
+    // ---------------- theta wave ----------------
+    const int lq = lane / NB, ub = lane - lq * NB;
+    bool act[VPL];
+    int qv[VPL];
+    size_t ebase[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; k++) {
+      const int qq = k * VW + lq, q = wave * vpw + qq;
+      act[k] = (lq < VW) & (qq < vpw) & (q < nv);
+      qv[k] = act[k] ? q : 0;
+      ebase[k] = sidx(P, z, nv > 0 ? P.view_idx[vb + qv[k]] : 0, ub, 0);
+    }
+    const float *colt = P.colt;
+    const size_t fbase = (size_t)f * Nxy;
+    auto rec_at = [&](int pix, int k) -> size_t {
+      const int lin = pix != kNoPix ? (pix >> 16) * P.N_x + (pix & 0xffff) : 0;
+      return ((fbase + lin) * (size_t)P.colt_nv + qv[k]) * REC;
+    };
+    const int *lst = list + n0;
+    auto pix_at = [&](int raw, int k, int n) -> int {
+      return n < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+    };
+
+    // column entries in flight: ring slot j holds voxel n + D + j
+    float ra[C][VPL];
+    int rw[C][VPL];
+    bool rv[C];
+    // voxels n .. n + D: slot i holds voxel n + i (a, w0, the (e, w) rows
+    // as read, and the coefficients of the updates to replay on them:
+    // rc[i][t] belongs to voxel n + i - D + t)
+    float qa[D + 1][VPL], qe[D + 1][VPL][MZ], qw[D + 1][VPL][MZ], rc[D + 1][D][VPL];
+    int q0[D + 1][VPL];
+    float dxs[D];  // dx of voxels n - D .. n - 1
+    int rawc = lst[min(lane, cnt - 1)];
+    int rawn = lst[min(G + lane, cnt - 1)];
+    // prologue: the column entries of voxels 0 .. D + C - 1, then the rows
+    // of voxels 0 .. D - 1
+    {
+      float a0[D + C][VPL];
+      int w0[D + C][VPL];
+      bool v0[D + C];
+#pragma unroll
+      for (int j = 0; j < D + C; j++) {
+        const int p = pix_at(rawc, j, j);
+        v0[j] = p != kNoPix;
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          const size_t r = rec_at(p, k);
+          a0[j][k] = colt[r + ub];
+          w0[j][k] = __float_as_int(colt[r + NB]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < D; i++)
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          const int w = (act[k] & v0[i]) ? w0[i][k] : -1;
+          q0[i][k] = w;
+          qa[i][k] = w >= 0 ? a0[i][k] : 0.0f;
+          ld_rows<MZ>(P.e + ebase[k] + (size_t)max(w, 0) * MZ, qe[i][k]);
+          ld_rows<MZ>(P.w + ebase[k] + (size_t)max(w, 0) * MZ, qw[i][k]);
+#pragma unroll
+          for (int t = 0; t < D; t++) rc[i][t][k] = 0.0f;
+        }
+#pragma unroll
+      for (int j = 0; j < C; j++) {
+        rv[j] = v0[D + j];
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          ra[j][k] = a0[D + j][k];
+          rw[j][k] = w0[D + j][k];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < D; t++) dxs[t] = 0.0f;
+    }
+    int rawnn = rawn;
+    unsigned long long nent = 0;
+    const int ngroups = (cnt + G - 1) / G;
+    for (int g = 0; g < ngroups; g++) {
+#pragma unroll
+      for (int s = 0; s < G; s++) {
+        const int n = g * G + s;  // past cnt: an empty slot, A = 0
+        const int par = n & 1;
+        if (s == 0) rawnn = lst[min((g + 2) * G + lane, cnt - 1)];
+        // issue: the column entry of voxel n + D + C, the rows of voxel n + D
+        {
+          constexpr int so0 = D + C;
+          const int so = s + so0;
+          const int p = so < G ? pix_at(rawc, so, n + so0) : pix_at(rawn, so - G, n + so0);
+          float na[VPL];
+          int nw[VPL];
+#pragma unroll
+          for (int k = 0; k < VPL; k++) {
+            const size_t r = rec_at(p, k);
+            na[k] = colt[r + ub];
+            nw[k] = __float_as_int(colt[r + NB]);
+          }
+#pragma unroll
+          for (int k = 0; k < VPL; k++) {
+            const int w = (act[k] & rv[0]) ? rw[0][k] : -1;
+            q0[D][k] = w;
+            qa[D][k] = w >= 0 ? ra[0][k] : 0.0f;
+            ld_rows<MZ>(P.e + ebase[k] + (size_t)max(w, 0) * MZ, qe[D][k]);
+            ld_rows<MZ>(P.w + ebase[k] + (size_t)max(w, 0) * MZ, qw[D][k]);
+#pragma unroll
+            for (int t = 0; t < D; t++) rc[D][t][k] = 0.0f;
+          }
+#pragma unroll
+          for (int j = 0; j + 1 < C; j++) {
+            rv[j] = rv[j + 1];
+#pragma unroll
+            for (int k = 0; k < VPL; k++) {
+              ra[j][k] = ra[j + 1][k];
+              rw[j][k] = rw[j + 1][k];
+            }
+          }
+          rv[C - 1] = p != kNoPix;
+#pragma unroll
+          for (int k = 0; k < VPL; k++) {
+            ra[C - 1][k] = na[k];
+            rw[C - 1][k] = nw[k];
+          }
+        }
+        // voxel n's rows brought up to date: the updates of voxels n - D ..
+        // n - 1, in order
+        float ce[VPL][MZ];
+#pragma unroll
+        for (int k = 0; k < VPL; k++)
+#pragma unroll
+          for (int m = 0; m < MZ; m++) {
+            float e = qe[0][k][m];
+#pragma unroll
+            for (int t = 0; t < D; t++) e = fmaf(-rc[0][t][k], dxs[t], e);
+            ce[k][m] = e;
+          }
+        // voxel n's theta sums (XT_VoxUpdate.c:61-92) with the
+        // generalized-Huber weights (XT_VoxUpdate.c:66-80)
+        float cwk[VPL][MZ];
+        bool sel[VPL][MZ], allsel = true;
+#pragma unroll
+        for (int k = 0; k < VPL; k++)
+#pragma unroll
+          for (int m = 0; m < MZ; m++) {
+            sel[k][m] = huber_sel(P, ce[k][m], qw[0][k][m]);
+            cwk[k][m] = qw[0][k][m];
+            allsel &= sel[k][m];
+          }
+        if (__builtin_expect(__ballot(!allsel) != 0ull, 0)) {
+#pragma unroll
+          for (int k = 0; k < VPL; k++)
+#pragma unroll
+            for (int m = 0; m < MZ; m++) {
+              const float cw_out = P.Td * __builtin_amdgcn_sqrtf(qw[0][k][m]) *
+                                   __builtin_amdgcn_rcpf(fabsf(ce[k][m]));
+              cwk[k][m] = sel[k][m] ? qw[0][k][m] : cw_out;
+            }
+        }
+        double t1;
+        float t2;
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+#pragma unroll
+          for (int m = 0; m < MZ; m++) {
+            const float ac = qa[0][k] * cwk[k][m];
+            const double p1 = (double)(-ce[k][m] * ac);
+            const float p2 = qa[0][k] * ac;
+            t1 = (k == 0 && m == 0) ? p1 : t1 + p1;
+            t2 = (k == 0 && m == 0) ? p2 : t2 + p2;
+          }
+          nent += qa[0][k] > 0.0f ? MZ : 0;
+        }
+        wave_total_df(t1, t2);
+        if (lane == 0) {
+          S.t1[par][wave] = t1;
+          S.t2[par][wave] = t2;
+        }
+        // voxel n's A value at the bins this lane holds for voxels n + 1 ..
+        // n + D: the coefficient of voxel n's update on them
+#pragma unroll
+        for (int i = 1; i <= D; i++)
+#pragma unroll
+          for (int k = 0; k < VPL; k++) {
+            const int d = q0[i][k] + ub - q0[0][k];
+            const bool ov = (q0[i][k] >= 0) & (q0[0][k] >= 0) & (d >= 0) & (d < NB);
+            const float av = __shfl(qa[0][k], lq * NB + (ov ? d : 0));
+            rc[i][D - i][k] = ov ? av : 0.0f;
+          }
+        __syncthreads();
+        float V;
+        bool skip;
+        const float xn = solve_step<MAXTH, 0>(P, S, par, nth, &V, &skip);
+        const float dx = xn - V;
+#pragma unroll
+        for (int k = 0; k < VPL; k++) {
+          if (qa[0][k] != 0.0f) {
+            float o[MZ];
+#pragma unroll
+            for (int m = 0; m < MZ; m++) o[m] = fmaf(-qa[0][k], dx, ce[k][m]);
+            st_rows<MZ>(P.e + ebase[k] + (size_t)q0[0][k] * MZ, o);
+          }
+        }
+        // rotate the rings: voxel n + 1 becomes the current one
+#pragma unroll
+        for (int i = 0; i < D; i++)
+#pragma unroll
+          for (int k = 0; k < VPL; k++) {
+            qa[i][k] = qa[i + 1][k];
+            q0[i][k] = q0[i + 1][k];
+#pragma unroll
+            for (int m = 0; m < MZ; m++) {
+              qe[i][k][m] = qe[i + 1][k][m];
+              qw[i][k][m] = qw[i + 1][k][m];
+            }
+#pragma unroll
+            for (int t = 0; t < D; t++) rc[i][t][k] = rc[i + 1][t][k];
+          }
+#pragma unroll
+        for (int t = 0; t + 1 < D; t++) dxs[t] = dxs[t + 1];
+        dxs[D - 1] = dx;
+      }
+      rawc = rawn;
+      rawn = rawnn;
+    }
+    const double nsum = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
+  } else {
+    // ---------------- prior wave ----------------
+    prior_wave<MAXTH, true>(P, S, list, n0, cnt, f, z, iter, seq, count_mag, nth, lane);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    P.cell_ent[cell] += ne;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Multi-wavefront voxel update for wide columns: the coarsest multires
 // stages, whose columns (up to NBX bins x the slice's mz rows, e.g. 17 bins x
 // 8 rows at mult 8) exceed the column table's 9 bins.  A lane owns one view
@@ -1893,6 +2203,15 @@ template __global__ void k_icd_wpass<5, 2, 16, 11, true>(KP, const int2 *, int, 
 template __global__ void k_icd_wpass<3, 1, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_wpass<3, 1, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
 
+// the deep-pipeline global-e kernel: 3 voxels' rows in flight, column
+// entries 4 voxels further (the finest and second-finest stages; with two
+// views per lane the rings would spill); 1 and 7 rows for comparison
+template __global__ void k_icd_mwd<3, 1, 1, 8, 3, 4>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwd<3, 1, 1, 15, 3, 4>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwd<5, 2, 1, 8, 3, 4>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwd<5, 2, 1, 15, 3, 4>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwd<3, 1, 1, 8, 1, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwd<3, 1, 1, 8, 7, 1>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
