@@ -62,7 +62,10 @@ typedef struct {
                                      passes in groups alone, 8 = the
                                      deep-pipeline multi-wavefront kernel
                                      with e in global memory where it
-                                     applies (3- and 5-bin columns) */
+                                     applies (3- and 5-bin columns), 9 =
+                                     the batched finest-stage kernel
+                                     (3-bin columns, one detector row per
+                                     slice, up to 315 views per cell) */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront
@@ -219,7 +222,8 @@ enum {
   TB_K_WLOOP = 6,     /* k_icd_wloop: views in groups of passes */
   TB_K_MWD = 7,       /* k_icd_mwd: as k_icd_mwg, rows of the next voxels
                          in flight */
-  TB_KERNEL_FAMILIES = 8
+  TB_K_BAT = 8,       /* k_icd_bat: the finest stage in batches of voxels */
+  TB_KERNEL_FAMILIES = 12
 };
 typedef struct {
   double voxel_ms;

@@ -34,16 +34,16 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1, 2, 3, 4, 5, 6, 7, 8],
+@pytest.fixture(params=[0, 1, 2, 3, 4, 5, 6, 7, 8, 9],
                 ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal", "wide", "wpass",
-                     "wloop", "mwdeep"])
+                     "wloop", "mwdeep", "batched"])
 def kernel(request):
     """Voxel-update kernel: 0 = the fastest that applies, 1 = the one-voxel
     kernel everywhere, 2 = the one-wavefront LDS kernel, 3 = the
     multi-wavefront LDS kernel, 4 = the multi-wavefront kernel with e in
     global memory, 5 = the wide-column multi-wavefront kernels, 6 = the
     pass-based ones alone, 7 = the looped pass kernel alone, 8 = the
-    deep-pipeline global-e kernel."""
+    deep-pipeline global-e kernel, 9 = the batched finest-stage kernel."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -61,6 +61,7 @@ OPTION_FAMILIES = {
     6: {tb.K_WPASS},
     7: {tb.K_WLOOP},
     8: {tb.K_MWD},
+    9: {tb.K_BAT},
 }
 
 
@@ -421,6 +422,9 @@ def test_icd_kernel_variants(kernel, mxy, mz, n_views, n_frames):
                     "one-wavefront kernel's case)")
     if kernel == 6 and (mxy, n_views) in ((4, 1400), (8, 600)):
         pytest.skip("more passes than k_icd_wpass holds in registers (k_icd_wloop's case)")
+    if kernel == 9 and not ((mxy, mz) == (1, 1) and nv <= 315):
+        pytest.skip("the batched kernel serves the finest stage (3-bin columns, one row) with up "
+                    "to 315 views per cell")
     if kernel == 8 and not ((mxy, mz) == (1, 1) and nv <= 315 or (mxy, mz) == (2, 2) and nv <= 180):
         pytest.skip("the deep-pipeline kernel serves 3-bin columns of up to 315 views and 5-bin "
                     "x 2-row columns of up to 180 (one view per lane, 15 theta waves)")

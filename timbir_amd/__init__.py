@@ -50,13 +50,13 @@ class TbStats(C.Structure):
                 ("zero_skips", C.c_int64), ("entries", C.c_int64), ("alg_bytes", C.c_double),
                 ("lds_launches", C.c_int64),
                 ("global_launches", C.c_int64),
-                ("family_launches", C.c_int64 * 8),
+                ("family_launches", C.c_int64 * 12),
                 ("column_builds", C.c_int64), ("column_bins", C.c_int32),
                 ("reserved", C.c_int32)]
 
 
 # voxel-update kernel families (TB_K_* in timbir_dev.h)
-K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP, K_MWD = range(8)
+K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP, K_MWD, K_BAT = range(9)
 
 
 class TimbirError(RuntimeError):

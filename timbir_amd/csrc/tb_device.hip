@@ -34,6 +34,8 @@ template <int NB, int MZ, int VPL, int MAXTH, int TF>
 __global__ void k_icd_mwg(KP, const int2 *, int, int, int, int, int);
 template <int NB, int MZ, int VPL, int MAXTH, int D, int C>
 __global__ void k_icd_mwd(KP, const int2 *, int, int, int, int, int);
+template <int MAXTH>
+__global__ void k_icd_bat(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int MAXTH>
 __global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int PMAX, int MAXTH, bool HL>
@@ -975,7 +977,8 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
   else if (2 * st->mult_xy + 1 <= 5 && st->mult_z == 2) nb = 5;
   if (st->N_r < nb) nb = 0;
   bool launched = false;
-  bool deep = false;  // the launch below is k_icd_mwd
+  bool deep = false;     // the launch below is k_icd_mwd
+  bool batched = false;  // the launch below is k_icd_bat
   auto run_mw = [&](mw_fn fn, int colnb, int nth, int vpw, size_t lds, bool global) -> int {
     // the column table may be allocated now: take the parameters again
     const int ec = ensure_columns(st, colnb);
@@ -991,7 +994,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
                        chunk, seq, count_mag, vpw);
     if (global) st->global_launches++;
     else st->lds_launches++;
-    st->fam_launches[global ? (deep ? TB_K_MWD : TB_K_MWG) : TB_K_MW]++;
+    st->fam_launches[global ? (batched ? TB_K_BAT : deep ? TB_K_MWD : TB_K_MWG) : TB_K_MW]++;
     launched = true;
     return 0;
   };
@@ -1022,6 +1025,16 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     }
     const size_t lds = icd_mw_lds_bytes(nv, st->N_r, st->mult_z, nth * vpl * 64);
     if (fn && lds + 1024 <= st->lds_max && run_mw(fn, nb, nth, vpw, lds, false) < 0) return -1;
+  }
+  if (!launched && tf == 0 && g_opt.kernel == 9 && 2 * st->mult_xy + 1 <= 3 && st->mult_z == 1 &&
+      st->N_r >= 3) {
+    // the batched finest-stage kernel: one view per lane, 3-bin columns
+    const int cap = 64 / 3;
+    const int nth = (st->max_nv + cap - 1) / cap;
+    const int vpw = (st->max_nv + nth - 1) / nth;
+    mw_fn fn = nth <= 8 ? k_icd_bat<8> : nth <= 15 ? k_icd_bat<15> : nullptr;
+    batched = fn != nullptr;
+    if (fn && run_mw(fn, 3, nth, vpw, 0, true) < 0) return -1;
   }
   if (!launched && tf == 0 && g_opt.kernel == 8) {
     // the deep-pipeline global-e kernel: the lane layout of k_icd_mwg with

@@ -1591,6 +1591,631 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwd(KP P, const int2 *
 }
 
 // ---------------------------------------------------------------------------
+// k_icd_bat: the finest stage's voxel update (3-bin columns, one detector
+// row per slice, one view per lane) in rounds of K voxels of each cell's
+// sequence, with three barriers per round instead of one per voxel.
+//
+// The sweep is the sequential one of XT_VoxUpdate.c:43-115, in the same
+// order and with the same result up to the order of the theta sums.  A
+// round takes the error sinogram as it finds it (e0).  Voxel i's theta1 is
+//   theta1_i = -sum_j a_ij wt(e_j) e_j,  e_j = e0_j - sum_{m<i} a_mj dx_m
+// (wt the generalized-Huber weight).  An entry that no earlier voxel of the
+// round touches keeps e0; one that stays in the quadratic branch (wt = w)
+// changes theta1 by dx_m a_mj a_ij w_j and leaves theta2; so with
+//   G_mi = sum over those entries of a_mj a_ij w_j
+// theta1_i = theta1_i(e0) + sum_{m<i} dx_m G_mi and theta2_i = theta2_i(e0)
+// hold exactly whenever every entry voxel i shares with an earlier voxel of
+// the round is in the quadratic branch both at e0 and after their updates.
+//
+// Round: the theta lanes (view, bin offset) read each voxel's column-table
+// record whole (one round ahead), its (e, w) entry, and every other voxel's
+// A value at that entry, and fold the K theta1 (FP64), K theta2 and
+// K(K-1)/2 G partial sums so that each lane ends with one total; the waves'
+// totals meet in LDS (barrier 1).  The prior wave, which has meanwhile
+// evaluated the K voxels' qGGMRF surrogates (two voxels per pass), solves
+// the K voxels in order from those sums, re-evaluating the surrogate of a
+// voxel whose in-plane neighbour the round has just updated, and publishes
+// the steps (barrier 2).  Each theta lane then checks its shared entries
+// with those steps (barrier 3): from the first voxel with a shared entry
+// outside the quadratic branch on either side, the round continues one
+// voxel at a time with theta taken from the entries' current values (two
+// barriers each), the prior wave re-solving it.  Finally every theta lane
+// writes the final e of its entry if no later voxel of the round has a
+// nonzero A value there: e0 minus the round's updates of that entry in
+// order, fmaf(-a, dx, e) each, the sequential sweep's arithmetic.
+// ---------------------------------------------------------------------------
+constexpr int kBat = 4;    // voxels per round
+constexpr int kBatF = 16;  // FP32 sums per round: theta2 x 4, G x 6, padding
+
+struct BatIn {
+  double t1[16][kBat];          // per theta wave: theta1(e0) of the round's voxels
+  float f[16][kBatF];           // per theta wave: theta2 of each voxel, then G_mi
+  alignas(16) float dx[kBat];   // the round's steps
+  int bad[16];                  // per theta wave: voxels whose theta was not exact
+  unsigned long long nent[16];
+};
+
+// Folding sums: many values per lane become one total per lane.  Level L
+// pairs the values (2t, 2t + 1) across lanes that differ in one bit (L = 0:
+// lane bit 5, permlane32 swap; 1: bit 4, permlane16 swap; 2: bit 3, DPP
+// row_ror:8; 3: bit 2, DPP row_half_mirror (lane 7 - l within 8); 4: bit 1
+// and 5: bit 0, DPP quad perms): the lane with the bit clear keeps the sum
+// of value 2t, its partner that of value 2t + 1.  Once one value is left
+// the remaining levels add it up over the lanes that still differ.
+template <int L>
+__device__ __forceinline__ float fold_level(float a, float b) {
+  if constexpr (L <= 1) {
+    const auto r = L == 0 ? __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b),
+                                                             false, false)
+                          : __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b),
+                                                             false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    constexpr int CTRL = L == 2 ? 0x128 : L == 3 ? 0x141 : L == 4 ? 0x4E : 0xB1;
+    const bool hi = (threadIdx.x >> (5 - L)) & 1;
+    const float send = hi ? a : b, keep = hi ? b : a;
+    return keep + dpp_f<CTRL>(send);
+  }
+}
+template <int L>
+__device__ __forceinline__ float fold_all(float v) {
+  if constexpr (L <= 1) return swap_sum_f<L == 0>(v);
+  else return v + dpp_f<L == 2 ? 0x128 : L == 3 ? 0x141 : L == 4 ? 0x4E : 0xB1>(v);
+}
+template <int L>
+__device__ __forceinline__ double fold_level_d(double a, double b) {
+  static_assert(L <= 1, "the FP64 fold pairs across rows only");
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  const unsigned al = (unsigned)ua, ah = (unsigned)(ua >> 32);
+  const unsigned bl = (unsigned)ub, bh = (unsigned)(ub >> 32);
+  const auto lo = L == 0 ? __builtin_amdgcn_permlane32_swap(al, bl, false, false)
+                         : __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+  const auto hi = L == 0 ? __builtin_amdgcn_permlane32_swap(ah, bh, false, false)
+                         : __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+  const double x = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+  const double y = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+  return x + y;
+}
+// 16 FP32 values: lane l ends with the total of value
+// (l5 | l4 << 1 | l3 << 2 | l2 << 3) (lane bits), equal in the 4 lanes of a
+// quad
+__device__ __forceinline__ float fold16(const float (&v)[16]) {
+  float u[8], w[4], x[2];
+#pragma unroll
+  for (int t = 0; t < 8; t++) u[t] = fold_level<0>(v[2 * t], v[2 * t + 1]);
+#pragma unroll
+  for (int t = 0; t < 4; t++) w[t] = fold_level<1>(u[2 * t], u[2 * t + 1]);
+#pragma unroll
+  for (int t = 0; t < 2; t++) x[t] = fold_level<2>(w[2 * t], w[2 * t + 1]);
+  float y = fold_level<3>(x[0], x[1]);
+  y = fold_all<4>(y);
+  return fold_all<5>(y);
+}
+__device__ __forceinline__ int fold16_index(int lane) {
+  return ((lane >> 5) & 1) | ((lane >> 3) & 2) | ((lane >> 1) & 4) | ((lane << 1) & 8);
+}
+// 4 FP64 values: lane l ends with the total of value (l5 | l4 << 1), equal
+// in the 16 lanes of a row
+__device__ __forceinline__ double fold4d(const double (&v)[4]) {
+  const double u0 = fold_level_d<0>(v[0], v[1]), u1 = fold_level_d<0>(v[2], v[3]);
+  double y = fold_level_d<1>(u0, u1);
+  y += dpp_d<0xB1>(y);
+  y += dpp_d<0x4E>(y);
+  y += dpp_d<0x141>(y);
+  y += dpp_d<0x140>(y);
+  return y;
+}
+__device__ __forceinline__ int fold4d_index(int lane) { return ((lane >> 5) & 1) | ((lane >> 3) & 2); }
+
+// The qGGMRF surrogate (XT_ICD_update.c:134-183) of two voxels at once: the
+// lanes of each half-wave hold the 29 neighbour values of one voxel (lane
+// 13 + 32h the voxel itself, NaN for a neighbour outside the volume) and
+// the per-neighbour constants; pb and pbx end in every lane of the half.
+struct PriorLane {
+  float ng, nsq, nisqp, nc, two_c;
+  int ln;
+};
+__device__ __forceinline__ void prior_pair(const PriorLane &C, float nbv, float V, float &pb,
+                                           float &pbx) {
+  const bool vld = (nbv == nbv) & (C.ln != 13);
+  const float dlt = vld ? V - nbv : 0.0f;
+  const float t1 = __builtin_amdgcn_exp2f(0.8f * __builtin_amdgcn_logf(fabsf(dlt))) * C.nisqp;
+  const float t3 = C.nc + t1;
+  const float b = fmaf(1.2f, t1, C.two_c) * __builtin_amdgcn_rcpf(t3 * t3 * C.nsq);
+  const float gb = (vld ? C.ng : 0.0f) * b;
+  pb = gb;
+  pbx = -gb * dlt;
+  lo32_total_ff(pb, pbx);
+}
+
+template <int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_bat(KP P, const int2 *__restrict__ cells,
+                                                            int iter, int chunk, int seq,
+                                                            int count_mag, int vpw) {
+  constexpr int NB = 3, G = 8, K = kBat;
+  static_assert(G % K == 0 && K == 4, "whole rounds of 4 voxels per group");
+  constexpr int VW = 64 / NB;  // views of one lane layer
+  __shared__ BatIn S;
+  const int2 cl = cells[blockIdx.x];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int Nxy = P.N_x * P.N_y;
+  const int vb = P.view_off[f], nv = P.view_off[f + 1] - vb;
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0) return;
+  for (int t = threadIdx.x; t < 16 * kBat; t += blockDim.x) (&S.t1[0][0])[t] = 0.0;
+  for (int t = threadIdx.x; t < 16 * kBatF; t += blockDim.x) (&S.f[0][0])[t] = 0.0f;
+  if (threadIdx.x < 16) S.bad[threadIdx.x] = 0;
+  __syncthreads();
+  const int ngroups = (cnt + G - 1) / G;
+  // the first voxel of the round whose theta was not exact (K: none), the
+  // same in every wave
+  auto first_bad = [&]() -> int {
+    int m = 0;
+#pragma unroll
+    for (int w = 0; w < MAXTH; w++) m |= S.bad[w];
+    return m ? __builtin_ctz((unsigned)m) : K;
+  };
+  // a theta total over the waves: a fixed pairwise tree over MAXTH slots
+  // (absent waves hold zero), slot `col` of each wave's row
+  auto sum_t1 = [&](int col) -> double {
+    double v[MAXTH];
+#pragma unroll
+    for (int w = 0; w < MAXTH; w++) v[w] = S.t1[w][col];
+#pragma unroll
+    for (int m = MAXTH; m > 1; m = (m + 1) / 2) {
+      const int h = (m + 1) / 2;
+#pragma unroll
+      for (int w = 0; w + h < m; w++) v[w] = v[w] + v[w + h];
+    }
+    return v[0];
+  };
+  auto sum_f = [&](int col) -> float {
+    float v[MAXTH];
+#pragma unroll
+    for (int w = 0; w < MAXTH; w++) v[w] = S.f[w][col];
+#pragma unroll
+    for (int m = MAXTH; m > 1; m = (m + 1) / 2) {
+      const int h = (m + 1) / 2;
+#pragma unroll
+      for (int w = 0; w + h < m; w++) v[w] = v[w] + v[w + h];
+    }
+    return v[0];
+  };
+
+  if (wave < nth) {
+    // ---------------- theta wave ----------------
+    const int lq = lane / NB, ub = lane - lq * NB;
+    const int qq = lq, q = wave * vpw + qq;
+    const bool act = (lq < VW) & (qq < vpw) & (q < nv);
+    const int qv = act ? q : 0;
+    const size_t ebase = sidx(P, z, nv > 0 ? P.view_idx[vb + qv] : 0, ub, 0);
+    const float4 *colt = reinterpret_cast<const float4 *>(P.colt);
+    const size_t fbase = (size_t)f * Nxy;
+    auto rec_at = [&](int pix) -> size_t {
+      const int lin = pix != kNoPix ? (pix >> 16) * P.N_x + (pix & 0xffff) : 0;
+      return (fbase + lin) * (size_t)P.colt_nv + qv;
+    };
+    const int *lst = list + n0;
+    auto pix_at = [&](int raw, int k, int n) -> int {
+      return n < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+    };
+    // A value of a record at window offset d (0 outside the window)
+    auto rec_a = [](const float4 &r, int d) -> float {
+      const float v = d == 0 ? r.x : d == 1 ? r.y : r.z;
+      return (unsigned)d < (unsigned)NB ? v : 0.0f;
+    };
+    // theta terms of one entry (XT_VoxUpdate.c:61-92) with the
+    // generalized-Huber weight (XT_VoxUpdate.c:66-80)
+    auto terms = [&](float a, float e, float w, bool sel, double &p1, float &p2) {
+      const float cw = sel ? w : P.Td * __builtin_amdgcn_sqrtf(w) * __builtin_amdgcn_rcpf(fabsf(e));
+      const float ac = a * cw;
+      p1 = (double)(-e * ac);
+      p2 = a * ac;
+    };
+
+    // column records of the current round and of the next one
+    float4 rc[K], rn[K];
+    bool vc[K], vn[K];
+    int rawc = lst[min(lane, cnt - 1)];
+    int rawn = lst[min(G + lane, cnt - 1)];
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+      const int p = pix_at(rawc, i, i);
+      vc[i] = p != kNoPix;
+      rc[i] = colt[rec_at(p)];
+    }
+    int rawnn = rawn;
+    unsigned long long nent = 0;
+    for (int g = 0; g < ngroups; g++) {
+#pragma unroll
+      for (int bb = 0; bb < G / K; bb++) {
+        const int s0 = bb * K;  // the round's first voxel in the group
+        if (bb == 0) rawnn = lst[min((g + 2) * G + lane, cnt - 1)];
+        // this round's entries: window start, own A value, (e0, w)
+        int w0[K];
+        float a[K], e[K], wv[K];
+        size_t ea[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          w0[i] = (act & vc[i]) ? __float_as_int(rc[i].w) : -1;
+          a[i] = w0[i] >= 0 ? (ub == 0 ? rc[i].x : ub == 1 ? rc[i].y : rc[i].z) : 0.0f;
+          ea[i] = ebase + (size_t)max(w0[i], 0);
+          e[i] = P.e[ea[i]];
+          wv[i] = P.w[ea[i]];
+        }
+        // the next round's records
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          const int sn = s0 + K + i;
+          const int p = sn < G ? pix_at(rawc, sn, g * G + sn) : pix_at(rawn, sn - G, g * G + sn);
+          vn[i] = p != kNoPix;
+          rn[i] = colt[rec_at(p)];
+        }
+        // A value of voxel m at the entry this lane holds for voxel i
+        float cmi[K][K];
+        bool later[K], shared[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          later[i] = false;
+          shared[i] = false;
+#pragma unroll
+          for (int m = 0; m < K; m++) {
+            if (m == i) continue;
+            const int d = w0[i] + ub - w0[m];
+            const float c = (w0[i] >= 0) & (w0[m] >= 0) ? rec_a(rc[m], d) : 0.0f;
+            cmi[m][i] = c;
+            if (m > i) later[i] |= c != 0.0f;
+            else shared[i] |= c != 0.0f;
+          }
+        }
+        // theta sums of e0 and the pair sums G_mi
+        double t1[K];
+        float fv[kBatF];
+        bool sel0[K];
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          sel0[i] = huber_sel(P, e[i], wv[i]);
+          terms(a[i], e[i], wv[i], sel0[i], t1[i], fv[i]);
+          const float aq = sel0[i] ? a[i] * wv[i] : 0.0f;
+#pragma unroll
+          for (int m = 0; m < i; m++) fv[K + i * (i - 1) / 2 + m] = cmi[m][i] * aq;
+          nent += a[i] > 0.0f ? 1 : 0;
+        }
+#pragma unroll
+        for (int t = K + K * (K - 1) / 2; t < kBatF; t++) fv[t] = 0.0f;
+        const double t1s = fold4d(t1);
+        const float fs = fold16(fv);
+        if ((lane & 15) == 0) S.t1[wave][fold4d_index(lane)] = t1s;
+        if ((lane & 3) == 0) S.f[wave][fold16_index(lane)] = fs;
+        __syncthreads();  // 1: the round's sums are in LDS
+        __syncthreads();  // 2: the prior wave's steps are in LDS
+        float dx[K];
+        {
+          const float4 d4 = *reinterpret_cast<const float4 *>(S.dx);
+          dx[0] = d4.x;
+          dx[1] = d4.y;
+          dx[2] = d4.z;
+          dx[3] = d4.w;
+        }
+        // each shared entry in the quadratic branch before and after the
+        // earlier voxels' updates?
+        int badm = 0;
+#pragma unroll
+        for (int i = 1; i < K; i++) {
+          float v = e[i];
+#pragma unroll
+          for (int m = 0; m < i; m++) v = fmaf(-cmi[m][i], dx[m], v);
+          const bool bad = shared[i] & (!sel0[i] | !huber_sel(P, v, wv[i]));
+          badm |= __ballot(bad) != 0ull ? 1 << i : 0;
+        }
+        if (lane == 0) S.bad[wave] = badm;
+        __syncthreads();  // 3: the waves' checks are in LDS
+        const int ib = first_bad();
+        // from voxel ib on, one voxel at a time from the entries' current
+        // values
+#pragma unroll
+        for (int i = 1; i < K; i++) {
+          if (i < ib) continue;
+          {
+            const float4 d4 = *reinterpret_cast<const float4 *>(S.dx);
+            dx[0] = d4.x;
+            dx[1] = d4.y;
+            dx[2] = d4.z;
+            dx[3] = d4.w;
+          }
+          float v = e[i];
+#pragma unroll
+          for (int m = 0; m < i; m++) v = fmaf(-cmi[m][i], dx[m], v);
+          double p1;
+          float p2;
+          terms(a[i], v, wv[i], huber_sel(P, v, wv[i]), p1, p2);
+          wave_total_df(p1, p2);
+          if (lane == 0) {
+            S.t1[wave][0] = p1;
+            S.f[wave][0] = p2;
+          }
+          __syncthreads();  // the voxel's sums are in LDS
+          __syncthreads();  // its step is in LDS
+        }
+        {
+          const float4 d4 = *reinterpret_cast<const float4 *>(S.dx);
+          dx[0] = d4.x;
+          dx[1] = d4.y;
+          dx[2] = d4.z;
+          dx[3] = d4.w;
+        }
+        // the final e of each entry, written by the last voxel of the round
+        // with a nonzero A value there
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          if (a[i] != 0.0f && !later[i]) {
+            float v = e[i];
+#pragma unroll
+            for (int m = 0; m < i; m++) v = fmaf(-cmi[m][i], dx[m], v);
+            P.e[ea[i]] = fmaf(-a[i], dx[i], v);
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < K; i++) {
+          rc[i] = rn[i];
+          vc[i] = vn[i];
+        }
+      }
+      rawc = rawn;
+      rawn = rawnn;
+    }
+    const double nsum = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
+  } else {
+    // ---------------- prior wave ----------------
+    const int ln = lane & 31;
+    PriorLane C;
+    C.ln = ln;
+    int ndz = 0, ndy = 0, ndx = 0, ndt = 0;
+    C.ng = 0.0f;
+    C.nsq = 1.0f;
+    C.nisqp = 1.0f;
+    C.nc = 1.0f;
+    if (ln < 29) {
+      if (ln < 27) {
+        ndz = ln / 9 - 1;
+        ndy = (ln / 3) % 3 - 1;
+        ndx = ln % 3 - 1;
+        C.ng = P.sfilt[ln];
+        C.nsq = P.sSq, C.nisqp = 1.0f / P.sSqp, C.nc = P.c_s;
+      } else {
+        ndt = ln == 27 ? -1 : 1;
+        C.ng = P.tfilt;
+        C.nsq = P.sTq, C.nisqp = 1.0f / P.sTqp, C.nc = P.c_t;
+      }
+    }
+    C.two_c = 2.0f * C.nc;
+    // the neighbour this lane loads (lanes 0..28; a swap gives the upper
+    // half-wave another voxel's values)
+    const bool nlane = lane < 29;
+    const bool inplane = lane < 27 && ndz == 0 && lane != 13;
+    const int noff = ndy * 65536 + ndx;
+    const int zz = z + ndz, ff = f + ndt;
+    const bool zok = nlane && zz >= 0 && zz < P.N_z && ff >= 0 && ff < P.N_time;
+    const int zc = min(max(zz, 0), P.N_z - 1), fc = min(max(ff, 0), P.N_time - 1);
+    const int nplane = (fc * P.N_z + zc) * Nxy;
+    const int cbase = cell * Nxy;
+    const float qnan = __int_as_float(0x7fc00000);
+
+    int apix[G], anid[G], bpix[G], bnid[G];
+    float anb[G], bnb[G], bnv[G];
+    int pend = kNoPix;
+    auto fill = [&](int k, int pix) {
+      bpix[k] = pix;
+      const bool valid = pix != kNoPix;
+      const int yy = valid ? (pix >> 16) : 0, xx = valid ? (pix & 0xffff) : 0;
+      const int jj = yy + ndy, kk = xx + ndx;
+      const bool nok = valid & zok & (jj >= 0) & (jj < P.N_y) & (kk >= 0) & (kk < P.N_x);
+      bnb[k] = P.obj[(unsigned)(nok ? nplane + jj * P.N_x + kk : P.nobj + 64)];
+      bnid[k] = (valid & inplane) ? pix + noff : -2;
+      bnv[k] = qnan;
+    };
+    // one voxel's surrogate from the lower half-wave's lanes
+    auto prior_one = [&](float nbv, float &pb, float &pbx, float &V, bool &skip) {
+      V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
+      const bool anypos = (__ballot((ln != 13) & (nbv > 0.0f)) & 0xffffffffull) != 0ull;
+      float qb, qbx;
+      prior_pair(C, nbv, V, qb, qbx);
+      pb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), 0));
+      pbx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qbx), 0));
+      skip = (V == 0.0f) && iter > 1 && !anypos;
+    };
+    double vmacc = 0.0;
+    unsigned long long nupd = 0, nzero = 0;
+    int raw = list[n0 + min(lane, cnt - 1)];
+    wait_vm_all();
+    int pnext[G];
+#pragma unroll
+    for (int k = 0; k < G; k++) pnext[k] = k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      apix[k] = kNoPix;
+      anid[k] = -2;
+      anb[k] = qnan;
+    }
+    for (int g = 0; g <= ngroups; g++) {
+#pragma unroll
+      for (int k = 0; k < G; k++) fill(k, pnext[k]);
+      raw = list[n0 + min((g + 1) * G + lane, cnt - 1)];
+#pragma unroll
+      for (int k = 0; k < G; k++) pend = lane == G + k ? bpix[k] : pend;
+      float rx = 0.0f, rm = 0.0f;
+      int ridx = P.nobj + lane;
+      int rsk = -1;  // lane k: voxel k of the group skipped (1), updated (0), absent (-1)
+      // voxel s's step from its surrogate and theta sums; the object and
+      // update-map values and the neighbour patches of the voxels after it
+      auto step_voxel = [&](int s, float pb, float pbx, float V, bool skip, double T1,
+                            float T2, bool *redo, int s0) -> float {
+        const int pix = apix[s];
+        const float th1 = (float)T1;
+        const float num = P.alpha * (pbx - th1);
+        const float step = num * __builtin_amdgcn_rcpf(pb + T2);
+        const float xn = skip ? V : V + step;
+        const float dx = xn - V;
+        // (dy + 1) * 65536 + (dx + 1) with |dy|, |dx| <= 1 for a neighbour
+        const unsigned dd = (unsigned)(pend - pix + 65537);
+        const bool adj = (pend != kNoPix) & (dd != 65537u) & ((dd & 0xffffu) <= 2u) &
+                         (dd <= 131074u);
+        if (__ballot(adj) != 0ull) {
+#pragma unroll
+          for (int o = 0; o < G; o++) {
+            if (o <= s) continue;
+            const bool hit = anid[o] == pix;
+            anb[o] = hit ? xn : anb[o];
+            if (o < s0 + K) redo[o - s0] |= __ballot(hit) != 0ull;
+          }
+#pragma unroll
+          for (int o = 0; o < G; o++) bnv[o] = bnid[o] == pix ? xn : bnv[o];
+        }
+        const int lin = cbase + (pix >> 16) * P.N_x + (pix & 0xffff);
+        rx = lane == s ? xn : rx;
+        rm = lane == s ? (skip ? 0.0f : fabsf(dx)) : rm;
+        ridx = lane == s ? lin : ridx;
+        rsk = lane == s ? (skip ? 1 : 0) : rsk;
+        return dx;
+      };
+      if (g > 0) {
+#pragma unroll
+        for (int bb = 0; bb < G / K; bb++) {
+          const int s0 = bb * K;
+          // the round's surrogates, two voxels per pass
+          float pb[K], pbx[K], V[K];
+          bool skip[K], redo[K];
+#pragma unroll
+          for (int i = 0; i < K; i += 2) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(anb[s0 + i]),
+                                                             __float_as_uint(anb[s0 + i + 1]),
+                                                             false, false);
+            const float nbv = __uint_as_float(sw[0]);  // lanes 32..63: voxel i + 1's lanes 0..31
+            const float Va = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
+            const float Vb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 45));
+            const unsigned long long pos = __ballot((ln != 13) & (nbv > 0.0f));
+            float qb, qbx;
+            prior_pair(C, nbv, lane < 32 ? Va : Vb, qb, qbx);
+            pb[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), 0));
+            pbx[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qbx), 0));
+            pb[i + 1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), 32));
+            pbx[i + 1] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qbx), 32));
+            V[i] = Va;
+            V[i + 1] = Vb;
+            skip[i] = (Va == 0.0f) && iter > 1 && (pos & 0xffffffffull) == 0ull;
+            skip[i + 1] = (Vb == 0.0f) && iter > 1 && (pos >> 32) == 0ull;
+          }
+#pragma unroll
+          for (int i = 0; i < K; i++) redo[i] = false;
+          __syncthreads();  // 1: the theta waves' sums are in LDS
+          // lane t < 16: FP32 total t (theta2 of voxel t, then G); lane i < 4
+          // also theta1 of voxel i
+          const float fa = sum_f(lane & 15);
+          const double da = sum_t1(lane & 3);
+          float T2[K], Gp[K * (K - 1) / 2], dxs[K];
+          double T1[K];
+#pragma unroll
+          for (int i = 0; i < K; i++) {
+            T2[i] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fa), i));
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(da);
+            const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)bits, i);
+            const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(bits >> 32), i);
+            T1[i] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+          }
+#pragma unroll
+          for (int t = 0; t < K * (K - 1) / 2; t++)
+            Gp[t] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fa), K + t));
+          // the round's voxels in order (XT_VoxUpdate.c:43-115)
+#pragma unroll
+          for (int i = 0; i < K; i++) {
+            const int s = s0 + i;
+            dxs[i] = 0.0f;
+            if (apix[s] == kNoPix) continue;  // only past the end of the sequence
+            if (redo[i]) prior_one(anb[s], pb[i], pbx[i], V[i], skip[i]);
+            double t = T1[i];
+#pragma unroll
+            for (int m = 0; m < i; m++) t += (double)dxs[m] * (double)Gp[i * (i - 1) / 2 + m];
+            dxs[i] = step_voxel(s, pb[i], pbx[i], V[i], skip[i], t, T2[i], redo, s0);
+          }
+          if (lane == 0) *reinterpret_cast<float4 *>(S.dx) = make_float4(dxs[0], dxs[1], dxs[2], dxs[3]);
+          __syncthreads();  // 2: the steps are in LDS
+          __syncthreads();  // 3: the theta waves' checks are in LDS
+          const int ib = first_bad();
+          // from voxel ib on, one voxel at a time: theta from the entries'
+          // current values, the surrogate from the neighbours' current values
+#pragma unroll
+          for (int i = 1; i < K; i++) {
+            if (i < ib) continue;
+            const int s = s0 + i;
+            __syncthreads();  // the voxel's sums are in LDS
+            const double T1i = sum_t1(0);
+            const float T2i = sum_f(0);
+            if (apix[s] != kNoPix) {
+              float pbi, pbxi, Vi;
+              bool ski;
+              prior_one(anb[s], pbi, pbxi, Vi, ski);
+              const float dx = step_voxel(s, pbi, pbxi, Vi, ski, T1i, T2i, redo, s0);
+              if (lane == 0) S.dx[i] = dx;
+            }
+            __syncthreads();  // its step is in LDS
+          }
+        }
+      }
+      // the group's update counts and |x| sums (updateVoxels_Atten's return
+      // value, XT_VoxUpdate.c:263-281), lane by lane
+      vmacc += (count_mag && rsk == 0) ? fabs((double)rx) : 0.0;
+      nupd += (unsigned long long)__popcll(__ballot(rsk == 0));
+      nzero += (unsigned long long)__popcll(__ballot(rsk == 1));
+      wait_vm_all();
+      P.obj[(unsigned)ridx] = rx;
+      P.mag[(unsigned)ridx] = rm;
+#pragma unroll
+      for (int k = 0; k < G; k++) {
+        apix[k] = bpix[k];
+        anid[k] = bnid[k];
+        anb[k] = bnv[k] == bnv[k] ? bnv[k] : bnb[k];
+      }
+      // B moves to lanes 0..G-1
+      pend = __shfl(pend, (lane + G) & 63);
+      pend = lane < G ? pend : kNoPix;
+#pragma unroll
+      for (int k = 0; k < G; k++)
+        pnext[k] = (g + 1) * G + k < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
+    }
+    const double voxmag = wave_sum_d(vmacc);
+    if (lane == 0 && seq != 2) {
+      P.cell_voxmag[cell] += voxmag;
+      P.cell_upd[cell] += nupd;
+      P.cell_zero[cell] += nzero;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    P.cell_ent[cell] += ne;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Multi-wavefront voxel update for wide columns: the coarsest multires
 // stages, whose columns (up to NBX bins x the slice's mz rows, e.g. 17 bins x
 // 8 rows at mult 8) exceed the column table's 9 bins.  A lane owns one view
@@ -2212,6 +2837,9 @@ template __global__ void k_icd_mwd<5, 2, 1, 8, 3, 4>(KP, const int2 *, int, int,
 template __global__ void k_icd_mwd<5, 2, 1, 15, 3, 4>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwd<3, 1, 1, 8, 1, 2>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwd<3, 1, 1, 8, 7, 1>(KP, const int2 *, int, int, int, int, int);
+// the batched finest-stage kernel: up to 8 or 15 theta waves
+template __global__ void k_icd_bat<8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_bat<15>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
