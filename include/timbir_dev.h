@@ -241,6 +241,38 @@ typedef struct {
   int32_t reserved;
 } tb_stats;
 int tb_get_stats(tb_stage *st, tb_stats *out);
+
+/* Batches of independent problems (no counterpart in the reference, whose
+ * concurrency unit is one problem's cells, XT_ICD_update.c:977-991): the
+ * stages of many scans advanced one ICD iteration at a time in lockstep on
+ * one stream.  Each voxel-update phase is one launch per kernel
+ * configuration over the cells of every member that uses it (members whose
+ * kernel has no multi-problem form launch their own); a member's kernels,
+ * arguments and order are those of tb_icd_iteration, so its results are
+ * bit for bit the same.  With TB_BATCH_GRAPH the launches of iterations 2
+ * on are captured once in a HIP graph and replayed while the members and
+ * their kernel choices stay the same.  Voxel-kernel time is not split per
+ * member (tb_stats.voxel_ms stays 0 for batched launches); the batch keeps
+ * its own totals. */
+typedef struct tb_batch tb_batch;
+enum { TB_BATCH_GRAPH = 1 };
+typedef struct {
+  int64_t iterations;    /* tb_batch_iteration calls */
+  int64_t launches;      /* voxel-update launches made */
+  int64_t captures;      /* graphs captured */
+  int64_t graph_replays; /* iterations run by replaying a graph */
+  int64_t groups;        /* multi-problem launch groups of the last call */
+  int64_t solo;          /* members launching their own kernel (last call) */
+  double ms;             /* GPU time of the iterations' launches */
+} tb_batch_stats;
+int tb_batch_create(tb_batch **out, int32_t flags);
+void tb_batch_destroy(tb_batch *b);
+/* One ICD iteration (tb_icd_iteration) of each of the n stages; converged
+ * and ratio (n entries each, may be NULL) receive each member's stop flag
+ * and ratio.  The stages must be distinct and on the batch's device. */
+int tb_batch_iteration(tb_batch *b, tb_stage *const *stages, int32_t n, int32_t iter,
+                       int32_t *converged, double *ratio);
+int tb_batch_get_stats(const tb_batch *b, tb_batch_stats *out);
 int tb_reset_stats(tb_stage *st);
 int tb_synchronize(tb_stage *st);
 

@@ -26,6 +26,8 @@ EXPORTS = [
     "tb_variance_update", "tb_voxel_update", "tb_update_colour", "tb_vsc_select",
     "tb_nhicd_list", "tb_icd_iteration", "tb_stage_run", "tb_get_stats", "tb_reset_stats",
     "tb_synchronize", "tb_stage_object_device",
+    # many problems in lockstep
+    "tb_batch_create", "tb_batch_destroy", "tb_batch_iteration", "tb_batch_get_stats",
     # learned initialiser kernels (tb_unet.hip)
     "tb_nn_last_error", "tb_dev_malloc", "tb_dev_free", "tb_dev_copy", "tb_dev_zero",
     "tb_dev_synchronize", "tb_nn_conv3d", "tb_nn_conv3d_flip", "tb_nn_conv3d_wgrad",
@@ -54,6 +56,14 @@ class TbStats(C.Structure):
                 ("column_builds", C.c_int64), ("column_bins", C.c_int32),
                 ("reserved", C.c_int32)]
 
+
+class TbBatchStats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("launches", C.c_int64), ("captures", C.c_int64),
+                ("graph_replays", C.c_int64), ("groups", C.c_int64), ("solo", C.c_int64),
+                ("ms", C.c_double)]
+
+
+BATCH_GRAPH = 1  # TB_BATCH_GRAPH
 
 # voxel-update kernel families (TB_K_* in timbir_dev.h)
 K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP, K_MWD, K_BAT = range(9)
@@ -109,6 +119,12 @@ def lib() -> C.CDLL:
     L.tb_reset_stats.argtypes = [vp]
     L.tb_synchronize.argtypes = [vp]
     L.tb_stage_object_device.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_uint64)]
+    L.tb_batch_create.argtypes = [C.POINTER(vp), i32]
+    L.tb_batch_destroy.argtypes = [vp]
+    L.tb_batch_destroy.restype = None
+    L.tb_batch_iteration.argtypes = [vp, C.POINTER(vp), i32, i32, C.POINTER(i32),
+                                     C.POINTER(C.c_double)]
+    L.tb_batch_get_stats.argtypes = [vp, C.POINTER(TbBatchStats)]
     L.reconstruct.restype = C.c_int
     L.reconstruct.argtypes = ([C.POINTER(C.POINTER(C.c_float))] + [vp] * 5 + [i32] * 4
                               + [C.c_float] * 7 + [i32] * 2 + [C.c_float] * 2
@@ -303,6 +319,46 @@ class Stage:
         p, n = C.c_void_p(), C.c_uint64()
         _check(self.L.tb_stage_object_device(self.h, C.byref(p), C.byref(n)))
         return p.value, n.value
+
+
+class Batch:
+    """Stages of independent problems advanced in lockstep (tb_batch): one
+    voxel-update launch per kernel configuration and phase over all
+    members' cells, iterations 2 on replayed from a HIP graph when `graph`.
+    A member's results equal, bit for bit, those of its own
+    Stage.icd_iteration."""
+
+    def __init__(self, graph: bool = True):
+        self.L = lib()
+        h = C.c_void_p()
+        _check(self.L.tb_batch_create(C.byref(h), BATCH_GRAPH if graph else 0))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tb_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def iteration(self, stages, it: int):
+        """One ICD iteration of every stage; returns (converged, ratio)
+        arrays, one entry per stage."""
+        n = len(stages)
+        arr = (C.c_void_p * n)(*[s.h.value for s in stages])
+        conv = (C.c_int32 * n)()
+        ratio = (C.c_double * n)()
+        _check(self.L.tb_batch_iteration(self.h, arr, n, it, conv, ratio))
+        return np.array(conv[:], np.int32), np.array(ratio[:], np.float64)
+
+    def stats(self) -> dict:
+        s = TbBatchStats()
+        _check(self.L.tb_batch_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in TbBatchStats._fields_}
 
 
 def reconstruct(scan: dict, restart: int = 0, debug=None):

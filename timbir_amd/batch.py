@@ -11,6 +11,15 @@ reconstruct() returns for that scan alone (no files are written).
 How many streams the GPU runs side by side is capped by the hardware queues
 HIP gives the process: GPU_MAX_HW_QUEUES (4 unless set before the first HIP
 call) bounds the useful number of workers.
+
+reconstruct_batched() instead advances all scans in lockstep through one
+tb_batch (timbir_amd.Batch): the scans whose next stage has the coarsest
+(mult_xy, mult_z) run that stage together, one ICD iteration of every
+member per call, each voxel-update phase one launch per kernel
+configuration over all their cells (a single-frame scan with 4 slices has 2
+cells per colour; 64 of them fill 128 CUs in one launch), iterations 2 on
+replayed from a HIP graph.  Every member keeps its own stop rule and drops
+out when it fires; results equal reconstruct_stages(), bit for bit.
 """
 from __future__ import annotations
 
@@ -86,3 +95,77 @@ def reconstruct_many(scans, workers: int = 4, progress=None) -> list:
             if progress:
                 progress(k)
     return out
+
+
+def reconstruct_batched(scans, graph: bool = True, progress=None, stats: dict | None = None) -> list:
+    """reconstruct_stages() for every scan, the scans advanced in lockstep
+    (see the module docstring).  Results in input order; progress(k), when
+    given, is called as scan k finishes; stats, when given, receives the
+    batch's counters (timbir_amd.Batch.stats) and the stage groups run."""
+    import timbir_amd as tb
+    n = len(scans)
+    scheds = [multires_schedule(s["proj_rows"], s["proj_cols"]) for s in scans]
+    max_iter = tb.get_options().max_iterations
+    nxt = [0] * n
+    prev = [None] * n
+    var = [1.0] * n
+    out = [None] * n
+    groups = []
+    b = tb.Batch(graph=graph)
+    live = {}
+    try:
+        while True:
+            pend = [k for k in range(n) if nxt[k] < len(scheds[k])]
+            if not pend:
+                break
+            # the coarsest stage any scan still has to run, for every scan
+            # whose next stage it is
+            key = max(scheds[k][nxt[k]] for k in pend)
+            members = [k for k in pend if scheds[k][nxt[k]] == key]
+            for k in members:
+                s = nxt[k]
+                mxy, mz = scheds[k][s]
+                st = tb.Stage(scans[k], mult_xy=mxy, mult_z=mz, stage_idx=s,
+                              prev_mult_z=scheds[k][s - 1][1] if s else 0, var_est=var[k])
+                live[k] = st
+                if prev[k] is not None:
+                    st.init_from_stage(prev[k])
+                    st.set_offset(prev[k].offset())
+                    prev[k].close()
+                    prev[k] = None
+                st.init_error_sino()
+            active = list(members)
+            it = 1
+            iters = {}
+            while active and it <= max_iter:
+                conv, _ = b.iteration([live[k] for k in active], it)
+                still = []
+                for k, c in zip(active, conv):
+                    if c and it > 1:
+                        iters[k] = it
+                    else:
+                        still.append(k)
+                active = still
+                it += 1
+            for k in active:
+                iters[k] = max_iter
+            groups.append({"mult": list(key), "members": len(members),
+                           "iterations": max(iters.values())})
+            for k in members:
+                st = live.pop(k)
+                if nxt[k] < len(scheds[k]) - 1:
+                    prev[k], var[k] = st, st.var_est
+                else:
+                    out[k] = st.object()
+                    st.close()
+                    if progress:
+                        progress(k)
+                nxt[k] += 1
+        if stats is not None:
+            stats.update(b.stats())
+            stats["stages"] = groups
+        return out
+    finally:
+        for st in list(live.values()) + [p for p in prev if p is not None]:
+            st.close()
+        b.close()

@@ -36,6 +36,16 @@ template <int NB, int MZ, int VPL, int MAXTH, int D, int C>
 __global__ void k_icd_mwd(KP, const int2 *, int, int, int, int, int);
 template <int MAXTH>
 __global__ void k_icd_bat(KP, const int2 *, int, int, int, int, int);
+template <int NB, int MZ, int VPL, int MAXTH>
+__global__ void k_icd_mw_m(const KP *, const int4 *, int, int, int, int);
+template <int NB, int MZ, int VPL, int MAXTH, int TF>
+__global__ void k_icd_mwg_m(const KP *, const int4 *, int, int, int, int);
+template <int NBX, int R, int MAXTH>
+__global__ void k_icd_wide_m(const KP *, const int4 *, int, int, int, int);
+template <int NBX, int R, int PMAX, int MAXTH, bool HL>
+__global__ void k_icd_wpass_m(const KP *, const int4 *, int, int, int, int);
+template <int NBX, int R, int GP, int MAXTH, bool HL>
+__global__ void k_icd_wloop_m(const KP *, const int4 *, int, int, int, int);
 template <int NBX, int R, int MAXTH>
 __global__ void k_icd_wide(KP, const int2 *, int, int, int, int, int);
 template <int NBX, int R, int PMAX, int MAXTH, bool HL>
@@ -124,6 +134,7 @@ struct tb_stage {
   size_t lds_max = 65536;
   int2 *cells[2] = {nullptr, nullptr};
   int ncells[2] = {0, 0};
+  std::vector<int2> cells_h[2];  // the same lists on the host
   double *cell_voxmag = nullptr;
   unsigned long long *cell_upd = nullptr, *cell_zero = nullptr, *cell_ent = nullptr;
   double *off_pnum = nullptr, *off_pden = nullptr, *off_num = nullptr, *off_den = nullptr;
@@ -443,6 +454,7 @@ static int stage_create_impl(tb_stage *st, const float *projections, const float
     if (!cl.empty())
       HIPCHK(hipMemcpyAsync(st->cells[c], cl.data(), cl.size() * sizeof(int2),
                             hipMemcpyHostToDevice, st->stream));
+    st->cells_h[c] = cl;
   }
   HIPCHK(hipMemcpyAsync(st->hr, hr_h.data(), hr_h.size() * sizeof(float), hipMemcpyHostToDevice,
                         st->stream));
@@ -954,16 +966,34 @@ static int ensure_columns(tb_stage *st, int nb) {
   return 0;
 }
 
+// The voxel-update kernel launch_update picks for a stage and phase, for
+// callers that launch it themselves (the multi-problem batch):
+//   kind 0: (KP, cells, iter, chunk, seq, count_mag, extra), block `block`
+//           threads, `lds` bytes of dynamic LDS, family TB_K_*;
+//   kind 1: the one-wavefront LDS kernel (KP, cells, iter, chunk, seq,
+//           count_mag);
+//   kind 2: the one-voxel kernel (KP, cells, iter, chunk, seq, pix,
+//           count_mag).
+struct VoxPlan {
+  const void *fn = nullptr;
+  int kind = 0, block = 0, extra = 0, family = 0;
+  size_t lds = 0;
+  bool global = true;
+};
+
 static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk,
-                         int seq, int pix, int count_mag) {
+                         int seq, int pix, int count_mag, VoxPlan *plan = nullptr) {
   if (ncells == 0) return 0;
   KP P = st->kp();
-  hipEvent_t *evp = ev_pair(st);
-  if (!evp) {
-    if (harvest_events(st)) return -1;
+  hipEvent_t *evp = nullptr;
+  if (!plan) {
     evp = ev_pair(st);
+    if (!evp) {
+      if (harvest_events(st)) return -1;
+      evp = ev_pair(st);
+    }
+    HIPCHK(hipEventRecord(evp[0], st->stream));
   }
-  HIPCHK(hipEventRecord(evp[0], st->stream));
   // kernel 0 (auto): the multi-wavefront LDS kernel, else the
   // multi-wavefront kernel with e in global memory, else the one-wavefront
   // LDS kernel, else the one-voxel kernel; 1: one-voxel only; 2:
@@ -990,13 +1020,42 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
     if (lds)
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int fam = global ? (batched ? TB_K_BAT : deep ? TB_K_MWD : TB_K_MWG) : TB_K_MW;
+    launched = true;
+    if (plan) {
+      plan->fn = reinterpret_cast<const void *>(fn);
+      plan->kind = 0;
+      plan->block = 64 * (nth + 1);
+      plan->extra = vpw;
+      plan->family = fam;
+      plan->lds = lds;
+      plan->global = global;
+      return 0;
+    }
     hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
                        chunk, seq, count_mag, vpw);
     if (global) st->global_launches++;
     else st->lds_launches++;
-    st->fam_launches[global ? (batched ? TB_K_BAT : deep ? TB_K_MWD : TB_K_MWG) : TB_K_MW]++;
-    launched = true;
+    st->fam_launches[fam]++;
     return 0;
+  };
+  // the same for the wide-column and pass kernels
+  auto run_wide = [&](mw_fn fn, int nth, int extra, size_t lds, int fam) {
+    launched = true;
+    if (plan) {
+      plan->fn = reinterpret_cast<const void *>(fn);
+      plan->kind = 0;
+      plan->block = 64 * (nth + 1);
+      plan->extra = extra;
+      plan->family = fam;
+      plan->lds = lds;
+      plan->global = true;
+      return;
+    }
+    hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
+                       chunk, seq, count_mag, extra);
+    st->global_launches++;
+    st->fam_launches[fam]++;
   };
   // FP32 theta sums (theta_fp 1 or 2) exist in the global-e kernel only
   const int tf = g_opt.theta_fp;
@@ -1144,11 +1203,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
-                         chunk, seq, count_mag, mz / R);
-      st->global_launches++;
-      st->fam_launches[TB_K_WIDE]++;
-      launched = true;
+      run_wide(fn, nth, mz / R, lds, TB_K_WIDE);
     }
   }
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5 || g_opt.kernel == 6) &&
@@ -1187,11 +1242,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
-                         chunk, seq, count_mag, vpw);
-      st->global_launches++;
-      st->fam_launches[TB_K_WPASS]++;
-      launched = true;
+      run_wide(fn, nth, vpw, lds, TB_K_WPASS);
     }
   }
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5 || g_opt.kernel == 7) &&
@@ -1217,11 +1268,7 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      hipLaunchKernelGGL(fn, dim3(ncells), dim3(64 * (nth + 1)), lds, st->stream, P, cells, iter,
-                         chunk, seq, count_mag, vpw);
-      st->global_launches++;
-      st->fam_launches[TB_K_WLOOP]++;
-      launched = true;
+      run_wide(fn, nth, vpw, lds, TB_K_WLOOP);
     }
   }
 
@@ -1237,12 +1284,34 @@ static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, 
       if (seq == 0 && ensure_order(st, iter)) return -1;
       HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fast),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      launched = true;
+      if (plan) {
+        plan->fn = reinterpret_cast<const void *>(fast);
+        plan->kind = 1;
+        plan->block = 64;
+        plan->family = TB_K_LDS;
+        plan->lds = lds;
+        plan->global = false;
+        return 0;
+      }
       hipLaunchKernelGGL(fast, dim3(ncells), dim3(64), lds, st->stream, P, cells, iter, chunk,
                          seq, count_mag);
       st->lds_launches++;
       st->fam_launches[TB_K_LDS]++;
-      launched = true;
     }
+  }
+  if (plan) {
+    if (!launched) {
+      plan->kind = 2;
+      plan->block = st->max_nv <= 64 ? 64 : st->max_nv <= 128 ? 128 : 256;
+      plan->fn = st->max_nv <= 64    ? reinterpret_cast<const void *>(k_update_voxels<64>)
+                 : st->max_nv <= 128 ? reinterpret_cast<const void *>(k_update_voxels<128>)
+                                     : reinterpret_cast<const void *>(k_update_voxels<256>);
+      plan->family = TB_K_ONE_VOXEL;
+      plan->lds = 0;
+      plan->global = false;
+    }
+    return 0;
   }
   if (!launched) st->fam_launches[TB_K_ONE_VOXEL]++;
   if (launched) {
@@ -1328,6 +1397,27 @@ static int maybe_offset(tb_stage *st) {
   return 0;
 }
 
+// convergence rule (XT_ICD_update.c:1066-1104): the masked update
+// magnitudes of the iteration against |x| of the last chunk's voxels
+static int convergence_check(tb_stage *st, int32_t *converged, double *ratio) {
+  const int ncell = st->N_time * st->N_z;
+  KP P = st->kp();
+  const int nb = std::min(st->red_n, grid_for(st->nobj()));
+  hipLaunchKernelGGL(k_mag_sum, dim3(nb), dim3(256), 0, st->stream, P, st->red);
+  HIPCHK(hipGetLastError());
+  double avg;
+  if (reduce_partials(st, nb, &avg)) return -1;
+  std::vector<double> vm(ncell);
+  if (d2h(st, vm.data(), st->cell_voxmag, ncell * sizeof(double))) return -1;
+  double vox_mag = 0;
+  for (double v : vm) vox_mag += v;
+  if (harvest_events(st)) return -1;
+  const double pct = 100 * avg / vox_mag;
+  if (ratio) *ratio = pct;
+  if (converged) *converged = pct < st->convg_thresh ? 1 : 0;
+  return 0;
+}
+
 extern "C" int tb_icd_iteration(tb_stage *st, int32_t iter, int32_t *converged, double *ratio) {
   const int ncell = st->N_time * st->N_z;
   for (int j = 0; j < kNhicdK; j++) {
@@ -1347,22 +1437,7 @@ extern "C" int tb_icd_iteration(tb_stage *st, int32_t iter, int32_t *converged, 
     }
   }
   if (st->updateVar == 1 && tb_variance_update(st)) return -1;
-  // convergence rule (XT_ICD_update.c:1066-1104)
-  KP P = st->kp();
-  const int nb = std::min(st->red_n, grid_for(st->nobj()));
-  hipLaunchKernelGGL(k_mag_sum, dim3(nb), dim3(256), 0, st->stream, P, st->red);
-  HIPCHK(hipGetLastError());
-  double avg;
-  if (reduce_partials(st, nb, &avg)) return -1;
-  std::vector<double> vm(ncell);
-  if (d2h(st, vm.data(), st->cell_voxmag, ncell * sizeof(double))) return -1;
-  double vox_mag = 0;
-  for (double v : vm) vox_mag += v;
-  if (harvest_events(st)) return -1;
-  const double pct = 100 * avg / vox_mag;
-  if (ratio) *ratio = pct;
-  if (converged) *converged = pct < st->convg_thresh ? 1 : 0;
-  return 0;
+  return convergence_check(st, converged, ratio);
 }
 
 extern "C" int tb_stage_run(tb_stage *st, int32_t max_iter, double *costs, int32_t *iters) {
@@ -1391,6 +1466,379 @@ extern "C" int tb_stage_run(tb_stage *st, int32_t max_iter, double *costs, int32
   }
   if (iters) *iters = it > max_iter ? max_iter : it;
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent problems advanced in lockstep.  Each member is a
+// stage of its own scan; tb_batch_iteration runs one ICD iteration
+// (tb_icd_iteration's sequence) of every member at once on the batch's
+// stream: each voxel-update phase is one launch per kernel configuration
+// over the cells of all members that use it (the kernels' multi-problem
+// entry points read the member's parameters from a device table), with the
+// members' offset, selection and fill kernels between them.  A member's
+// kernels, arguments and order are those of its own tb_icd_iteration, so
+// its result is bit for bit the same.  From the second iteration on the
+// launches of an iteration are captured once in a HIP graph and replayed
+// (the kernels depend on the iteration only through iter > 1; the voxel
+// order of each iteration is built before the replay).
+// ---------------------------------------------------------------------------
+namespace {
+const void *multi_of(const void *fn) {
+#define TB_FN(...) reinterpret_cast<const void *>(__VA_ARGS__)
+  static const std::pair<const void *, const void *> tab[] = {
+      {TB_FN(k_icd_mw<3, 1, 1, 4>), TB_FN(k_icd_mw_m<3, 1, 1, 4>)},
+      {TB_FN(k_icd_mw<3, 1, 2, 2>), TB_FN(k_icd_mw_m<3, 1, 2, 2>)},
+      {TB_FN(k_icd_mw<3, 1, 2, 3>), TB_FN(k_icd_mw_m<3, 1, 2, 3>)},
+      {TB_FN(k_icd_mw<3, 1, 1, 8>), TB_FN(k_icd_mw_m<3, 1, 1, 8>)},
+      {TB_FN(k_icd_mw<3, 1, 1, 15>), TB_FN(k_icd_mw_m<3, 1, 1, 15>)},
+      {TB_FN(k_icd_mw<5, 2, 1, 8>), TB_FN(k_icd_mw_m<5, 2, 1, 8>)},
+      {TB_FN(k_icd_mw<5, 2, 1, 15>), TB_FN(k_icd_mw_m<5, 2, 1, 15>)},
+      {TB_FN(k_icd_mw<5, 2, 2, 3>), TB_FN(k_icd_mw_m<5, 2, 2, 3>)},
+      {TB_FN(k_icd_wide<17, 1, 8>), TB_FN(k_icd_wide_m<17, 1, 8>)},
+      {TB_FN(k_icd_wide<17, 2, 8>), TB_FN(k_icd_wide_m<17, 2, 8>)},
+      {TB_FN(k_icd_wloop<17, 2, 8, 15, false>), TB_FN(k_icd_wloop_m<17, 2, 8, 15, false>)},
+      {TB_FN(k_icd_wloop<9, 2, 8, 15, false>), TB_FN(k_icd_wloop_m<9, 2, 8, 15, false>)},
+      {TB_FN(k_icd_wloop<5, 2, 8, 15, false>), TB_FN(k_icd_wloop_m<5, 2, 8, 15, false>)},
+      {TB_FN(k_icd_wloop<3, 1, 8, 15, false>), TB_FN(k_icd_wloop_m<3, 1, 8, 15, false>)},
+      {TB_FN(k_icd_wpass<17, 2, 16, 11, false>), TB_FN(k_icd_wpass_m<17, 2, 16, 11, false>)},
+      {TB_FN(k_icd_wpass<17, 2, 24, 11, false>), TB_FN(k_icd_wpass_m<17, 2, 24, 11, false>)},
+      {TB_FN(k_icd_wpass<9, 2, 16, 11, false>), TB_FN(k_icd_wpass_m<9, 2, 16, 11, false>)},
+      {TB_FN(k_icd_wpass<5, 2, 16, 11, false>), TB_FN(k_icd_wpass_m<5, 2, 16, 11, false>)},
+      {TB_FN(k_icd_wpass<17, 2, 16, 11, true>), TB_FN(k_icd_wpass_m<17, 2, 16, 11, true>)},
+      {TB_FN(k_icd_wpass<17, 2, 24, 11, true>), TB_FN(k_icd_wpass_m<17, 2, 24, 11, true>)},
+      {TB_FN(k_icd_wpass<9, 2, 16, 11, true>), TB_FN(k_icd_wpass_m<9, 2, 16, 11, true>)},
+      {TB_FN(k_icd_wpass<5, 2, 16, 11, true>), TB_FN(k_icd_wpass_m<5, 2, 16, 11, true>)},
+      {TB_FN(k_icd_wpass<3, 1, 16, 11, false>), TB_FN(k_icd_wpass_m<3, 1, 16, 11, false>)},
+      {TB_FN(k_icd_wpass<3, 1, 16, 11, true>), TB_FN(k_icd_wpass_m<3, 1, 16, 11, true>)},
+      {TB_FN(k_icd_mwg<3, 1, 1, 8, 0>), TB_FN(k_icd_mwg_m<3, 1, 1, 8, 0>)},
+      {TB_FN(k_icd_mwg<3, 1, 1, 15, 0>), TB_FN(k_icd_mwg_m<3, 1, 1, 15, 0>)},
+      {TB_FN(k_icd_mwg<5, 2, 1, 8, 0>), TB_FN(k_icd_mwg_m<5, 2, 1, 8, 0>)},
+      {TB_FN(k_icd_mwg<5, 2, 1, 15, 0>), TB_FN(k_icd_mwg_m<5, 2, 1, 15, 0>)},
+      {TB_FN(k_icd_mwg<9, 4, 1, 15, 0>), TB_FN(k_icd_mwg_m<9, 4, 1, 15, 0>)},
+      {TB_FN(k_icd_mwg<9, 4, 2, 15, 0>), TB_FN(k_icd_mwg_m<9, 4, 2, 15, 0>)},
+      {TB_FN(k_icd_mwg<3, 1, 2, 15, 0>), TB_FN(k_icd_mwg_m<3, 1, 2, 15, 0>)},
+      {TB_FN(k_icd_mwg<3, 1, 4, 11, 0>), TB_FN(k_icd_mwg_m<3, 1, 4, 11, 0>)},
+      {TB_FN(k_icd_mwg<5, 2, 2, 15, 0>), TB_FN(k_icd_mwg_m<5, 2, 2, 15, 0>)},
+      {TB_FN(k_icd_mwg<5, 2, 4, 11, 0>), TB_FN(k_icd_mwg_m<5, 2, 4, 11, 0>)},
+      {TB_FN(k_icd_mwg<9, 2, 1, 15, 0>), TB_FN(k_icd_mwg_m<9, 2, 1, 15, 0>)},
+      {TB_FN(k_icd_mwg<9, 2, 2, 15, 0>), TB_FN(k_icd_mwg_m<9, 2, 2, 15, 0>)},
+      {TB_FN(k_icd_mwg<9, 2, 4, 11, 0>), TB_FN(k_icd_mwg_m<9, 2, 4, 11, 0>)},
+  };
+#undef TB_FN
+  for (const auto &e : tab)
+    if (e.first == fn) return e.second;
+  return nullptr;
+}
+}  // namespace
+
+struct tb_batch {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int flags = 0;
+  KP *tab = nullptr;  // device: one parameter block per member
+  size_t tab_cap = 0;
+  int4 *cells = nullptr;  // device: the multi launches' cell lists
+  size_t cells_cap = 0;
+  std::vector<KP> tab_h;
+  std::vector<int4> cells_h;
+  // launch groups: a multi-problem kernel, its block and LDS, and the
+  // range of its cells per colour; solo members launch their own kernel
+  struct Group {
+    const void *multi;
+    int block, family;
+    size_t lds;
+    bool global;
+    std::vector<int> members;
+    size_t off[2], n[2];
+  };
+  std::vector<Group> groups;
+  std::vector<int> solo;
+  std::vector<VoxPlan> plans;
+  std::vector<tb_stage *> members;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool graph_ok = false;
+  tb_batch_stats stats{};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static void batch_drop_graph(tb_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_ok = false;
+}
+
+extern "C" int tb_batch_create(tb_batch **out, int32_t flags) {
+  *out = nullptr;
+  auto *b = new tb_batch();
+  b->flags = flags;
+  if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreate(&b->stream) != hipSuccess ||
+      hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess) {
+    tb_batch_destroy(b);
+    return fail("tb_batch_create: HIP stream or events");
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" void tb_batch_destroy(tb_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  if (b->tab) (void)hipFree(b->tab);
+  if (b->cells) (void)hipFree(b->cells);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" int tb_batch_get_stats(const tb_batch *b, tb_batch_stats *out) {
+  *out = b->stats;
+  return 0;
+}
+
+// the voxel-update phase (colour c) of every member
+static int batch_phase(tb_batch *b, int c, int iter, int chunk, int seq, int count_mag) {
+  using mm_fn = void (*)(const KP *, const int4 *, int, int, int, int);
+  using mw_fn = void (*)(KP, const int2 *, int, int, int, int, int);
+  using lds_fn = void (*)(KP, const int2 *, int, int, int, int);
+  using one_fn = void (*)(KP, const int2 *, int, int, int, int, int);
+  for (const auto &g : b->groups) {
+    if (!g.n[c]) continue;
+    hipLaunchKernelGGL(reinterpret_cast<mm_fn>(const_cast<void *>(g.multi)), dim3((unsigned)g.n[c]),
+                       dim3(g.block), g.lds, b->stream, b->tab, b->cells + g.off[c], iter, chunk,
+                       seq, count_mag);
+  }
+  for (int i : b->solo) {
+    tb_stage *st = b->members[i];
+    const VoxPlan &pl = b->plans[i];
+    if (!st->ncells[c] || !pl.fn) continue;
+    const KP P = st->kp();
+    if (pl.kind == 0)
+      hipLaunchKernelGGL(reinterpret_cast<mw_fn>(const_cast<void *>(pl.fn)), dim3(st->ncells[c]),
+                         dim3(pl.block), pl.lds, b->stream, P, st->cells[c], iter, chunk, seq,
+                         count_mag, pl.extra);
+    else if (pl.kind == 1)
+      hipLaunchKernelGGL(reinterpret_cast<lds_fn>(const_cast<void *>(pl.fn)), dim3(st->ncells[c]),
+                         dim3(pl.block), pl.lds, b->stream, P, st->cells[c], iter, chunk, seq,
+                         count_mag);
+    else
+      hipLaunchKernelGGL(reinterpret_cast<one_fn>(const_cast<void *>(pl.fn)), dim3(st->ncells[c]),
+                         dim3(pl.block), 0, b->stream, P, st->cells[c], iter, chunk, seq, 0,
+                         count_mag);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// tb_icd_iteration's launches for every member, up to the variance update
+static int batch_body(tb_batch *b, int iter) {
+  const int n = (int)b->members.size();
+  for (int j = 0; j < kNhicdK; j++) {
+    for (tb_stage *st : b->members) {
+      const int ncell = st->N_time * st->N_z;
+      hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(ncell)), dim3(256), 0, b->stream,
+                         st->cell_voxmag, (size_t)ncell, 0.0);
+    }
+    if (batch_phase(b, 0, iter, j, 0, 1)) return -1;
+    for (tb_stage *st : b->members)
+      if (maybe_offset(st)) return -1;
+    if (batch_phase(b, 1, iter, j, 0, 1)) return -1;
+    for (tb_stage *st : b->members)
+      if (maybe_offset(st)) return -1;
+    for (tb_stage *st : b->members)
+      if (vsc_async(st)) return -1;
+    if (iter > 1 && g_opt.no_nhicd == 0) {
+      if (batch_phase(b, 0, iter, j, 1, 0)) return -1;
+      for (tb_stage *st : b->members)
+        if (maybe_offset(st)) return -1;
+      if (batch_phase(b, 1, iter, j, 1, 0)) return -1;
+      for (tb_stage *st : b->members)
+        if (maybe_offset(st)) return -1;
+    }
+  }
+  (void)n;
+  return 0;
+}
+
+// plans, launch groups, parameter table and cell lists of the members;
+// returns 1 when they differ from those the graph was captured with
+static int batch_prepare(tb_batch *b, tb_stage *const *sts, int n, int iter, bool *changed) {
+  std::vector<VoxPlan> plans(n);
+  for (int i = 0; i < n; i++) {
+    tb_stage *st = sts[i];
+    const int c = st->ncells[0] ? 0 : 1;
+    if (!st->ncells[c]) continue;
+    if (launch_update(st, st->cells[c], st->ncells[c], iter, 0, 0, 0, 1, &plans[i])) return -1;
+  }
+  std::vector<tb_batch::Group> groups;
+  std::vector<int> solo;
+  for (int i = 0; i < n; i++) {
+    const VoxPlan &pl = plans[i];
+    if (!pl.fn) continue;
+    const void *m = pl.kind == 0 ? multi_of(pl.fn) : nullptr;
+    if (!m) {
+      solo.push_back(i);
+      continue;
+    }
+    tb_batch::Group *g = nullptr;
+    for (auto &h : groups)
+      if (h.multi == m && h.block == pl.block) g = &h;
+    if (!g) {
+      groups.push_back({m, pl.block, pl.family, 0, pl.global, {}, {0, 0}, {0, 0}});
+      g = &groups.back();
+    }
+    g->lds = std::max(g->lds, pl.lds);
+    g->members.push_back(i);
+  }
+  std::vector<int4> cells;
+  for (auto &g : groups)
+    for (int c = 0; c < 2; c++) {
+      g.off[c] = cells.size();
+      for (int i : g.members) {
+        tb_stage *st = sts[i];
+        for (const int2 &q : st->cells_h[c]) cells.push_back(make_int4(q.x, q.y, i, plans[i].extra));
+      }
+      g.n[c] = cells.size() - g.off[c];
+    }
+  std::vector<KP> tab(n);
+  for (int i = 0; i < n; i++) tab[i] = sts[i]->kp();
+  *changed = b->members.size() != (size_t)n ||
+             !std::equal(b->members.begin(), b->members.end(), sts) ||
+             b->cells_h.size() != cells.size() ||
+             memcmp(b->cells_h.data(), cells.data(), cells.size() * sizeof(int4)) != 0 ||
+             b->tab_h.size() != tab.size() ||
+             memcmp(b->tab_h.data(), tab.data(), tab.size() * sizeof(KP)) != 0 ||
+             b->groups.size() != groups.size();
+  for (size_t k = 0; !*changed && k < groups.size(); k++)
+    *changed = groups[k].multi != b->groups[k].multi || groups[k].block != b->groups[k].block ||
+               groups[k].lds != b->groups[k].lds;
+  for (int i = 0; !*changed && i < n; i++)
+    *changed = plans[i].fn != b->plans[i].fn || plans[i].extra != b->plans[i].extra;
+  if (*changed) {
+    if (tab.size() > b->tab_cap) {
+      if (b->tab) (void)hipFree(b->tab);
+      b->tab = nullptr;
+      b->tab_cap = 0;
+      HIPCHK(dalloc(&b->tab, tab.size()));
+      b->tab_cap = tab.size();
+    }
+    if (cells.size() > b->cells_cap) {
+      if (b->cells) (void)hipFree(b->cells);
+      b->cells = nullptr;
+      b->cells_cap = 0;
+      HIPCHK(dalloc(&b->cells, std::max<size_t>(cells.size(), 1)));
+      b->cells_cap = std::max<size_t>(cells.size(), 1);
+    }
+    HIPCHK(hipMemcpyAsync(b->tab, tab.data(), tab.size() * sizeof(KP), hipMemcpyHostToDevice,
+                          b->stream));
+    if (!cells.empty())
+      HIPCHK(hipMemcpyAsync(b->cells, cells.data(), cells.size() * sizeof(int4),
+                            hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));  // the host copies stay valid until here
+    b->tab_h = std::move(tab);
+    b->cells_h = std::move(cells);
+    b->groups = std::move(groups);
+    b->solo = std::move(solo);
+    b->plans = std::move(plans);
+    b->members.assign(sts, sts + n);
+    for (const auto &g : b->groups)
+      if (g.lds)
+        HIPCHK(hipFuncSetAttribute(g.multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)g.lds));
+  }
+  return 0;
+}
+
+// the members' voxel-launch counters for one iteration
+static void batch_count(tb_batch *b, int iter) {
+  const int per = (iter > 1 && g_opt.no_nhicd == 0) ? 2 : 1;
+  for (size_t i = 0; i < b->members.size(); i++) {
+    tb_stage *st = b->members[i];
+    const VoxPlan &pl = b->plans[i];
+    if (!pl.fn) continue;
+    const int k = per * kNhicdK * ((st->ncells[0] ? 1 : 0) + (st->ncells[1] ? 1 : 0));
+    st->voxel_launches += k;
+    st->fam_launches[pl.family] += k;
+    if (pl.global) st->global_launches += k;
+    else st->lds_launches += k;
+  }
+  for (const auto &g : b->groups) b->stats.launches += per * kNhicdK * ((g.n[0] ? 1 : 0) + (g.n[1] ? 1 : 0));
+  for (int i : b->solo) {
+    tb_stage *st = b->members[i];
+    b->stats.launches += per * kNhicdK * ((st->ncells[0] ? 1 : 0) + (st->ncells[1] ? 1 : 0));
+  }
+}
+
+static int batch_iteration(tb_batch *b, tb_stage *const *sts, int n, int iter,
+                           int32_t *converged, double *ratio) {
+  bool changed = false;
+  if (batch_prepare(b, sts, n, iter, &changed)) return -1;
+  if (changed) batch_drop_graph(b);
+  HIPCHK(hipEventRecord(b->ev[0], b->stream));
+  const bool use_graph = (b->flags & TB_BATCH_GRAPH) && iter > 1;
+  if (use_graph && !b->graph_ok) {
+    batch_drop_graph(b);
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_body(b, iter);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return -1;
+    }
+    HIPCHK(ce);
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_ok = true;
+    b->stats.captures++;
+  }
+  if (use_graph) {
+    HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    b->stats.graph_replays++;
+  } else if (batch_body(b, iter)) {
+    return -1;
+  }
+  HIPCHK(hipEventRecord(b->ev[1], b->stream));
+  batch_count(b, iter);
+  for (int i = 0; i < n; i++) {
+    tb_stage *st = sts[i];
+    if (st->updateVar == 1 && tb_variance_update(st)) return -1;
+    if (convergence_check(st, converged ? converged + i : nullptr, ratio ? ratio + i : nullptr))
+      return -1;
+  }
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  b->stats.ms += ms;
+  b->stats.iterations++;
+  b->stats.groups = (int64_t)b->groups.size();
+  b->stats.solo = (int64_t)b->solo.size();
+  return 0;
+}
+
+extern "C" int tb_batch_iteration(tb_batch *b, tb_stage *const *stages, int32_t n, int32_t iter,
+                                  int32_t *converged, double *ratio) {
+  if (!b || n < 0 || (n && !stages)) return fail("tb_batch_iteration: bad arguments");
+  if (n == 0) return 0;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != b->device) return fail("tb_batch_iteration: the batch belongs to device %d", b->device);
+  // every member's work goes to the batch's stream for the call
+  std::vector<hipStream_t> own(n);
+  for (int i = 0; i < n; i++) {
+    for (int k = 0; k < i; k++)
+      if (stages[k] == stages[i]) return fail("tb_batch_iteration: stage %d given twice", i);
+    HIPCHK(hipStreamSynchronize(stages[i]->stream));
+    own[i] = stages[i]->stream;
+    stages[i]->stream = b->stream;
+  }
+  const int rc = batch_iteration(b, stages, n, iter, converged, ratio);
+  if (rc) (void)hipStreamSynchronize(b->stream);
+  for (int i = 0; i < n; i++) stages[i]->stream = own[i];
+  return rc;
 }
 
 extern "C" int tb_get_stats(tb_stage *st, tb_stats *out) {

@@ -713,16 +713,14 @@ __device__ __forceinline__ void prior_wave(const KP &P, StepIn &S, const int *li
 }
 
 template <int NB, int MZ, int VPL, int MAXTH>
-__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mw(KP P, const int2 *__restrict__ cells,
-                                                            int iter, int chunk, int seq,
-                                                            int count_mag, int vpw) {
+__device__ __forceinline__ void k_icd_mw_body(const KP &P, const int2 cl, int iter,
+                                             int chunk, int seq, int count_mag, int vpw) {
   constexpr int EPV = NB * MZ;  // entries of one view
   constexpr int VW = 64 / EPV;  // views of one lane layer
   constexpr int G = 8;          // voxels per prefetch group
   constexpr int REC = col_rec(NB);
   extern __shared__ float2 s_ew[];  // [nv][RS] (e, w), then one dummy pair per lane and layer
   __shared__ StepIn S;
-  const int2 cl = cells[blockIdx.x];
   const int f = cl.x, z = cl.y;
   const int cell = f * P.N_z + z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -987,15 +985,39 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mw(KP P, const int2 *_
   }
 }
 
+// one problem: the stage's cell list
+template <int NB, int MZ, int VPL, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mw(KP P, const int2 *__restrict__ cells,
+    int iter, int chunk, int seq, int count_mag, int vpw) {
+  k_icd_mw_body<NB, MZ, VPL, MAXTH>(P, cells[blockIdx.x], iter, chunk, seq, count_mag, vpw);
+}
+// many problems: cells (frame, slice, problem, last argument), parameters
+// from the problems' table
+template <int NB, int MZ, int VPL, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mw_m(const KP *__restrict__ tab,
+    const int4 *__restrict__ cells, int iter, int chunk, int seq, int count_mag) {
+  const int4 c = cells[blockIdx.x];
+  k_icd_mw_body<NB, MZ, VPL, MAXTH>(tab[c.z], make_int2(c.x, c.y), iter, chunk, seq, count_mag, c.w);
+}
+
+
 // views per wave: VW * VPL = 21 * VPL (3 bins x 1 row) or 6 * VPL (5 bins x 2 rows)
 template __global__ void k_icd_mw<3, 1, 1, 4>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<3, 1, 1, 4>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mw<3, 1, 2, 2>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<3, 1, 2, 2>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mw<3, 1, 2, 3>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<3, 1, 2, 3>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mw<3, 1, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<3, 1, 1, 8>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mw<3, 1, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<3, 1, 1, 15>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mw<5, 2, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<5, 2, 1, 8>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mw<5, 2, 1, 15>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<5, 2, 1, 15>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mw<5, 2, 2, 3>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mw_m<5, 2, 2, 3>(const KP *, const int4 *, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Multi-wavefront voxel update with the error sinogram left in global memory
@@ -1049,13 +1071,12 @@ __device__ __forceinline__ void st_rows(float *p, const float (&o)[MZ]) {
 }
 
 template <int NB, int MZ, int VPL, int MAXTH, int TF>
-__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *__restrict__ cells, int iter,
-                                                  int chunk, int seq, int count_mag, int vpw) {
+__device__ __forceinline__ void k_icd_mwg_body(const KP &P, const int2 cl, int iter,
+                                              int chunk, int seq, int count_mag, int vpw) {
   constexpr int G = 8;             // pixels read per list load
   constexpr int VW = 64 / NB;      // views of one lane layer
   constexpr int REC = col_rec(NB);
   __shared__ StepIn S;
-  const int2 cl = cells[blockIdx.x];
   const int f = cl.x, z = cl.y;
   const int cell = f * P.N_z + z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1293,6 +1314,22 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *
     P.cell_ent[cell] += ne;
   }
 }
+
+// one problem: the stage's cell list
+template <int NB, int MZ, int VPL, int MAXTH, int TF>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg(KP P, const int2 *__restrict__ cells,
+    int iter, int chunk, int seq, int count_mag, int vpw) {
+  k_icd_mwg_body<NB, MZ, VPL, MAXTH, TF>(P, cells[blockIdx.x], iter, chunk, seq, count_mag, vpw);
+}
+// many problems: cells (frame, slice, problem, last argument), parameters
+// from the problems' table
+template <int NB, int MZ, int VPL, int MAXTH, int TF>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_mwg_m(const KP *__restrict__ tab,
+    const int4 *__restrict__ cells, int iter, int chunk, int seq, int count_mag) {
+  const int4 c = cells[blockIdx.x];
+  k_icd_mwg_body<NB, MZ, VPL, MAXTH, TF>(tab[c.z], make_int2(c.x, c.y), iter, chunk, seq, count_mag, c.w);
+}
+
 
 // ---------------------------------------------------------------------------
 // k_icd_mwd: the global-e multi-wavefront update (k_icd_mwg's lane layout,
@@ -2240,12 +2277,10 @@ __device__ __forceinline__ void ld_r(const float *p, float (&o)[R]) {
 }
 
 template <int NBX, int R, int MAXTH>
-__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wide(KP P, const int2 *__restrict__ cells,
-                                                            int iter, int chunk, int seq,
-                                                            int count_mag, int ngr) {
+__device__ __forceinline__ void k_icd_wide_body(const KP &P, const int2 cl, int iter,
+                                               int chunk, int seq, int count_mag, int ngr) {
   __shared__ StepIn S;
   extern __shared__ float s_H[];  // [views of the frame][kHrStride]
-  const int2 cl = cells[blockIdx.x];
   const int f = cl.x, z = cl.y;
   const int cell = f * P.N_z + z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2381,9 +2416,27 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wide(KP P, const int2 
   }
 }
 
+// one problem: the stage's cell list
+template <int NBX, int R, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wide(KP P, const int2 *__restrict__ cells,
+    int iter, int chunk, int seq, int count_mag, int ngr) {
+  k_icd_wide_body<NBX, R, MAXTH>(P, cells[blockIdx.x], iter, chunk, seq, count_mag, ngr);
+}
+// many problems: cells (frame, slice, problem, last argument), parameters
+// from the problems' table
+template <int NBX, int R, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wide_m(const KP *__restrict__ tab,
+    const int4 *__restrict__ cells, int iter, int chunk, int seq, int count_mag) {
+  const int4 c = cells[blockIdx.x];
+  k_icd_wide_body<NBX, R, MAXTH>(tab[c.z], make_int2(c.x, c.y), iter, chunk, seq, count_mag, c.w);
+}
+
+
 // up to 8 theta waves: with more, the 17 x R (e, w) registers spill
 template __global__ void k_icd_wide<17, 1, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wide_m<17, 1, 8>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wide<17, 2, 8>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wide_m<17, 2, 8>(const KP *, const int4 *, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Multi-wavefront voxel update for wide columns and many views: the coarse
@@ -2406,13 +2459,11 @@ struct WideView {
 };
 
 template <int NBX, int R, int PMAX, int MAXTH, bool HL>
-__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2 *__restrict__ cells,
-                                                            int iter, int chunk, int seq,
-                                                            int count_mag, int vpw) {
+__device__ __forceinline__ void k_icd_wpass_body(const KP &P, const int2 cl, int iter,
+                                                int chunk, int seq, int count_mag, int vpw) {
   __shared__ StepIn S;
   // [views of the frame] coefficients, then (HL) their detector responses
   extern __shared__ WideView s_view[];
-  const int2 cl = cells[blockIdx.x];
   const int f = cl.x, z = cl.y;
   const int cell = f * P.N_z + z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2598,6 +2649,22 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
   }
 }
 
+// one problem: the stage's cell list
+template <int NBX, int R, int PMAX, int MAXTH, bool HL>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2 *__restrict__ cells,
+    int iter, int chunk, int seq, int count_mag, int vpw) {
+  k_icd_wpass_body<NBX, R, PMAX, MAXTH, HL>(P, cells[blockIdx.x], iter, chunk, seq, count_mag, vpw);
+}
+// many problems: cells (frame, slice, problem, last argument), parameters
+// from the problems' table
+template <int NBX, int R, int PMAX, int MAXTH, bool HL>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass_m(const KP *__restrict__ tab,
+    const int4 *__restrict__ cells, int iter, int chunk, int seq, int count_mag) {
+  const int4 c = cells[blockIdx.x];
+  k_icd_wpass_body<NBX, R, PMAX, MAXTH, HL>(tab[c.z], make_int2(c.x, c.y), iter, chunk, seq, count_mag, c.w);
+}
+
+
 // ---------------------------------------------------------------------------
 // k_icd_wloop: k_icd_wpass for any number of passes.  The wave walks its
 // entries in groups of GP passes (one group's e, w and A values in
@@ -2609,13 +2676,11 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wpass(KP P, const int2
 // registers); the detector responses stay in global memory.
 // ---------------------------------------------------------------------------
 template <int NBX, int R, int GP, int MAXTH, bool HL>
-__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wloop(KP P, const int2 *__restrict__ cells,
-                                                            int iter, int chunk, int seq,
-                                                            int count_mag, int vpw) {
+__device__ __forceinline__ void k_icd_wloop_body(const KP &P, const int2 cl, int iter,
+                                                int chunk, int seq, int count_mag, int vpw) {
   __shared__ StepIn S;
   // [views of the frame] coefficients, then (HL) their detector responses
   extern __shared__ WideView s_view[];
-  const int2 cl = cells[blockIdx.x];
   const int f = cl.x, z = cl.y;
   const int cell = f * P.N_z + z;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2809,24 +2874,54 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wloop(KP P, const int2
   }
 }
 
+// one problem: the stage's cell list
+template <int NBX, int R, int GP, int MAXTH, bool HL>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wloop(KP P, const int2 *__restrict__ cells,
+    int iter, int chunk, int seq, int count_mag, int vpw) {
+  k_icd_wloop_body<NBX, R, GP, MAXTH, HL>(P, cells[blockIdx.x], iter, chunk, seq, count_mag, vpw);
+}
+// many problems: cells (frame, slice, problem, last argument), parameters
+// from the problems' table
+template <int NBX, int R, int GP, int MAXTH, bool HL>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_wloop_m(const KP *__restrict__ tab,
+    const int4 *__restrict__ cells, int iter, int chunk, int seq, int count_mag) {
+  const int4 c = cells[blockIdx.x];
+  k_icd_wloop_body<NBX, R, GP, MAXTH, HL>(tab[c.z], make_int2(c.x, c.y), iter, chunk, seq, count_mag, c.w);
+}
+
+
 // 15 theta waves: a group of 8 passes fits the registers of 4 waves per SIMD
 template __global__ void k_icd_wloop<17, 2, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wloop_m<17, 2, 8, 15, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wloop<9, 2, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wloop_m<9, 2, 8, 15, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wloop<5, 2, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wloop_m<5, 2, 8, 15, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wloop<3, 1, 8, 15, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wloop_m<3, 1, 8, 15, false>(const KP *, const int4 *, int, int, int, int);
 
 // 11 theta waves keep the passes' registers unspilled
 template __global__ void k_icd_wpass<17, 2, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<17, 2, 16, 11, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<17, 2, 24, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<17, 2, 24, 11, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<9, 2, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<9, 2, 16, 11, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<5, 2, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<5, 2, 16, 11, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<17, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<17, 2, 16, 11, true>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<17, 2, 24, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<17, 2, 24, 11, true>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<9, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<9, 2, 16, 11, true>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<5, 2, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<5, 2, 16, 11, true>(const KP *, const int4 *, int, int, int, int);
 // single-row slices (the finest stage) with more views than k_icd_mwg holds
 template __global__ void k_icd_wpass<3, 1, 16, 11, false>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<3, 1, 16, 11, false>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_wpass<3, 1, 16, 11, true>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_wpass_m<3, 1, 16, 11, true>(const KP *, const int4 *, int, int, int, int);
 
 // the deep-pipeline global-e kernel: 3 voxels' rows in flight, column
 // entries 4 voxels further (the finest and second-finest stages; with two
@@ -2841,20 +2936,33 @@ template __global__ void k_icd_mwd<3, 1, 1, 8, 7, 1>(KP, const int2 *, int, int,
 template __global__ void k_icd_bat<8>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_bat<15>(KP, const int2 *, int, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<3, 1, 1, 8, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<3, 1, 1, 15, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 1, 8, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<5, 2, 1, 8, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<5, 2, 1, 15, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<9, 4, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<9, 4, 1, 15, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<9, 4, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<9, 4, 2, 15, 0>(const KP *, const int4 *, int, int, int, int);
 // more views per lane for cells with many views (single-frame scans); with
 // 4 views per lane the registers fit only 11 theta waves without spilling
 template __global__ void k_icd_mwg<3, 1, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<3, 1, 2, 15, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<3, 1, 4, 11, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<3, 1, 4, 11, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<5, 2, 2, 15, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<5, 2, 4, 11, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<5, 2, 4, 11, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<9, 2, 1, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<9, 2, 1, 15, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<9, 2, 2, 15, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<9, 2, 2, 15, 0>(const KP *, const int4 *, int, int, int, int);
 template __global__ void k_icd_mwg<9, 2, 4, 11, 0>(KP, const int2 *, int, int, int, int, int);
+template __global__ void k_icd_mwg_m<9, 2, 4, 11, 0>(const KP *, const int4 *, int, int, int, int);
 
 // FP32 theta sums, compensated (TF 1) and plain (TF 2), for the 3- and 5-bin
 // columns (the finest and second-finest stages)
