@@ -91,6 +91,10 @@ typedef struct {
                                      flight while one is solved, in the
                                      deep-pipeline kernel: 0 = 3
                                      (default), 1, 3 or 7 */
+  int32_t autotune;               /* 1 = with kernel 0, use the kernel
+                                     configuration the autotuner measured
+                                     fastest for the stage's shape, timing
+                                     it on first use (default 0) */
 } tb_options;
 
 void tb_get_options(tb_options *opt);
@@ -273,6 +277,26 @@ void tb_batch_destroy(tb_batch *b);
 int tb_batch_iteration(tb_batch *b, tb_stage *const *stages, int32_t n, int32_t iter,
                        int32_t *converged, double *ratio);
 int tb_batch_get_stats(const tb_batch *b, tb_batch_stats *out);
+
+/* Autotuner (no counterpart in the reference): per stage shape, the kernel
+ * option (2-9) and views per lane whose HICD colour phase ran fastest on
+ * this GPU, timed on the stage's own data with every trial undone.
+ * tb_autotune tunes the stage's shape (or returns the cached choice);
+ * tb_autotune_entries copies up to cap cached entries and returns how many
+ * there are; tb_autotune_put adds one (a saved cache); tb_autotune_clear
+ * empties the cache.  With tb_options.autotune, launches of kernel 0 use
+ * it. */
+typedef struct {
+  int32_t shape[8]; /* mult_xy, mult_z, views per frame, N_r, N_x, N_y, N_z, N_time */
+  int32_t kernel;   /* tb_options.kernel of the fastest configuration */
+  int32_t views_per_lane;
+  float ms;         /* its time for one HICD colour phase */
+  int32_t reserved;
+} tb_tune_entry;
+int tb_autotune(tb_stage *st, tb_tune_entry *out);
+int tb_autotune_entries(tb_tune_entry *out, int32_t cap);
+int tb_autotune_put(const tb_tune_entry *e);
+void tb_autotune_clear(void);
 int tb_reset_stats(tb_stage *st);
 int tb_synchronize(tb_stage *st);
 

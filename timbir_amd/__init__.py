@@ -28,6 +28,8 @@ EXPORTS = [
     "tb_synchronize", "tb_stage_object_device",
     # many problems in lockstep
     "tb_batch_create", "tb_batch_destroy", "tb_batch_iteration", "tb_batch_get_stats",
+    # autotuner
+    "tb_autotune", "tb_autotune_entries", "tb_autotune_put", "tb_autotune_clear",
     # learned initialiser kernels (tb_unet.hip)
     "tb_nn_last_error", "tb_dev_malloc", "tb_dev_free", "tb_dev_copy", "tb_dev_zero",
     "tb_dev_synchronize", "tb_nn_conv3d", "tb_nn_conv3d_flip", "tb_nn_conv3d_wgrad",
@@ -43,7 +45,7 @@ class TbOptions(C.Structure):
                 ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32),
                 ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
                 ("init_fbp", C.c_int32), ("no_column_table", C.c_int32),
-                ("theta_fp", C.c_int32), ("row_lookahead", C.c_int32)]
+                ("theta_fp", C.c_int32), ("row_lookahead", C.c_int32), ("autotune", C.c_int32)]
 
 
 class TbStats(C.Structure):
@@ -64,6 +66,11 @@ class TbBatchStats(C.Structure):
 
 
 BATCH_GRAPH = 1  # TB_BATCH_GRAPH
+
+
+class TbTuneEntry(C.Structure):
+    _fields_ = [("shape", C.c_int32 * 8), ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
+                ("ms", C.c_float), ("reserved", C.c_int32)]
 
 # voxel-update kernel families (TB_K_* in timbir_dev.h)
 K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP, K_MWD, K_BAT = range(9)
@@ -125,6 +132,11 @@ def lib() -> C.CDLL:
     L.tb_batch_iteration.argtypes = [vp, C.POINTER(vp), i32, i32, C.POINTER(i32),
                                      C.POINTER(C.c_double)]
     L.tb_batch_get_stats.argtypes = [vp, C.POINTER(TbBatchStats)]
+    L.tb_autotune.argtypes = [vp, C.POINTER(TbTuneEntry)]
+    L.tb_autotune_entries.argtypes = [C.POINTER(TbTuneEntry), i32]
+    L.tb_autotune_put.argtypes = [C.POINTER(TbTuneEntry)]
+    L.tb_autotune_clear.argtypes = []
+    L.tb_autotune_clear.restype = None
     L.reconstruct.restype = C.c_int
     L.reconstruct.argtypes = ([C.POINTER(C.POINTER(C.c_float))] + [vp] * 5 + [i32] * 4
                               + [C.c_float] * 7 + [i32] * 2 + [C.c_float] * 2
@@ -319,6 +331,57 @@ class Stage:
         p, n = C.c_void_p(), C.c_uint64()
         _check(self.L.tb_stage_object_device(self.h, C.byref(p), C.byref(n)))
         return p.value, n.value
+
+    def autotune(self) -> dict:
+        """The kernel configuration the autotuner finds fastest for this
+        stage's shape (timed now unless cached), as a dict."""
+        e = TbTuneEntry()
+        _check(self.L.tb_autotune(self.h, C.byref(e)))
+        return _tune_dict(e)
+
+
+def _tune_dict(e) -> dict:
+    return {"shape": list(e.shape), "kernel": e.kernel, "views_per_lane": e.views_per_lane,
+            "ms": e.ms}
+
+
+def autotune_entries() -> list:
+    """The autotuner's cached choices, one dict per stage shape."""
+    L = lib()
+    n = L.tb_autotune_entries(None, 0)
+    arr = (TbTuneEntry * max(n, 1))()
+    n = L.tb_autotune_entries(arr, n)
+    return [_tune_dict(arr[i]) for i in range(n)]
+
+
+def autotune_put(entries) -> None:
+    """Load choices (dicts as autotune_entries returns) into the cache."""
+    L = lib()
+    for d in entries:
+        e = TbTuneEntry()
+        for i, v in enumerate(d["shape"]):
+            e.shape[i] = int(v)
+        e.kernel = int(d["kernel"])
+        e.views_per_lane = int(d["views_per_lane"])
+        e.ms = float(d.get("ms", 0.0))
+        _check(L.tb_autotune_put(C.byref(e)))
+
+
+def autotune_clear() -> None:
+    lib().tb_autotune_clear()
+
+
+def autotune_save(path: str) -> None:
+    """Write the cache as JSON (autotune_load reads it back)."""
+    import json
+    with open(path, "w") as f:
+        json.dump(autotune_entries(), f, indent=1)
+
+
+def autotune_load(path: str) -> None:
+    import json
+    with open(path) as f:
+        autotune_put(json.load(f))
 
 
 class Batch:

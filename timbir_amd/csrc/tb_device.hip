@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -79,7 +80,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -966,6 +967,8 @@ static int ensure_columns(tb_stage *st, int nb) {
   return 0;
 }
 
+static int autotune_pick(tb_stage *st, int iter, int *kernel, int *vpl);
+
 // The voxel-update kernel launch_update picks for a stage and phase, for
 // callers that launch it themselves (the multi-problem batch):
 //   kind 0: (KP, cells, iter, chunk, seq, count_mag, extra), block `block`
@@ -981,8 +984,28 @@ struct VoxPlan {
   bool global = true;
 };
 
+static int launch_update_as(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk,
+                            int seq, int pix, int count_mag, VoxPlan *plan);
+
+// With tb_options.autotune and kernel 0, the kernel and views per lane the
+// autotuner measured fastest for the stage's shape (timed on first use)
 static int launch_update(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk,
                          int seq, int pix, int count_mag, VoxPlan *plan = nullptr) {
+  if (ncells == 0) return 0;
+  if (!g_opt.autotune || g_opt.kernel != 0 || seq == 2)
+    return launch_update_as(st, cells, ncells, iter, chunk, seq, pix, count_mag, plan);
+  int k = 0, v = 0;
+  if (autotune_pick(st, iter, &k, &v)) return -1;
+  const tb_options saved = g_opt;
+  g_opt.kernel = k;
+  g_opt.views_per_lane = v;
+  const int rc = launch_update_as(st, cells, ncells, iter, chunk, seq, pix, count_mag, plan);
+  g_opt = saved;
+  return rc;
+}
+
+static int launch_update_as(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk,
+                            int seq, int pix, int count_mag, VoxPlan *plan) {
   if (ncells == 0) return 0;
   KP P = st->kp();
   hipEvent_t *evp = nullptr;
@@ -1839,6 +1862,189 @@ extern "C" int tb_batch_iteration(tb_batch *b, tb_stage *const *stages, int32_t 
   if (rc) (void)hipStreamSynchronize(b->stream);
   for (int i = 0; i < n; i++) stages[i]->stream = own[i];
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner (no counterpart in the reference): for a stage shape (mult_xy,
+// mult_z, views per frame, bins, image, slices, frames) it times one HICD
+// colour phase with every kernel configuration that applies (kernel options
+// 2-9, one or two views per lane for the multi-wavefront LDS kernel) and
+// keeps the fastest.  Each trial runs on the stage's own buffers and is
+// undone: the error sinogram, object, update map and per-cell counters are
+// copied aside before and back after it, and the stage's launch counters
+// and kernel time are restored, so tuning leaves the stage as it found it.
+// Every configuration runs the same sweep (the GPU tests hold each to the
+// oracle), but their sums round differently: with tuning on, which one ran
+// depends on the timings, so results are reproducible only with the same
+// cache (tb_autotune_put fills it from a saved one).
+// ---------------------------------------------------------------------------
+namespace {
+struct TuneKey {
+  int v[8];
+  bool operator<(const TuneKey &o) const { return memcmp(v, o.v, sizeof(v)) < 0; }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, tb_tune_entry> g_tune;
+
+TuneKey tune_key(const tb_stage *st) {
+  return {{st->mult_xy, st->mult_z, st->max_nv, st->N_r, st->N_x, st->N_y, st->N_z, st->N_time}};
+}
+}  // namespace
+
+static int autotune_run(tb_stage *st, int iter, tb_tune_entry *best) {
+  if (!st->ncells[0]) {
+    best->kernel = 0;
+    best->views_per_lane = 0;
+    best->ms = 0;
+    return 0;
+  }
+  if (harvest_events(st)) return -1;
+  // state the trials change, copied aside
+  const int ncell = st->N_time * st->N_z;
+  const size_t ns = st->nsino(), no = st->nobj();
+  float *e0 = nullptr, *x0 = nullptr, *m0 = nullptr;
+  double *c0 = nullptr;
+  unsigned long long *u0 = nullptr;
+  auto release = [&]() {
+    for (void *p : {(void *)e0, (void *)x0, (void *)m0, (void *)c0, (void *)u0})
+      if (p) (void)hipFree(p);
+  };
+  if (dalloc(&e0, ns) != hipSuccess || dalloc(&x0, no) != hipSuccess ||
+      dalloc(&m0, no) != hipSuccess || dalloc(&c0, ncell) != hipSuccess ||
+      dalloc(&u0, 3 * (size_t)ncell) != hipSuccess) {
+    release();
+    return fail("autotune: no memory for the trial copies");
+  }
+  auto copy = [&](bool save) -> hipError_t {
+    auto cp = [&](void *a, void *b, size_t n) {
+      return hipMemcpyAsync(save ? a : b, save ? b : a, n, hipMemcpyDeviceToDevice, st->stream);
+    };
+    hipError_t r = cp(e0, st->e, ns * sizeof(float));
+    if (r == hipSuccess) r = cp(x0, st->obj, no * sizeof(float));
+    if (r == hipSuccess) r = cp(m0, st->mag, no * sizeof(float));
+    if (r == hipSuccess) r = cp(c0, st->cell_voxmag, ncell * sizeof(double));
+    if (r == hipSuccess) r = cp(u0, st->cell_upd, ncell * 8);
+    if (r == hipSuccess) r = cp(u0 + ncell, st->cell_zero, ncell * 8);
+    if (r == hipSuccess) r = cp(u0 + 2 * (size_t)ncell, st->cell_ent, ncell * 8);
+    return r;
+  };
+  const double vms = st->voxel_ms;
+  const long long vl = st->voxel_launches, ll = st->lds_launches, gl = st->global_launches;
+  long long fam[TB_KERNEL_FAMILIES];
+  memcpy(fam, st->fam_launches, sizeof(fam));
+  const tb_options saved = g_opt;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = 0;
+  best->ms = -1;
+  std::vector<const void *> seen;
+  if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess ||
+      copy(true) != hipSuccess)
+    rc = fail("autotune: events or copies");
+  const int kernels[] = {3, 4, 5, 6, 7, 8, 9, 2};
+  for (int k : kernels) {
+    for (int v = (k == 3 ? 1 : 0); v <= (k == 3 ? 2 : 0) && !rc; v++) {
+      g_opt = saved;
+      g_opt.autotune = 0;
+      g_opt.kernel = k;
+      g_opt.views_per_lane = v;
+      VoxPlan pl;
+      if (launch_update_as(st, st->cells[0], st->ncells[0], iter, 0, 0, 0, 1, &pl)) {
+        rc = -1;
+        break;
+      }
+      if (!pl.fn || pl.kind != 0 ||
+          std::find(seen.begin(), seen.end(), pl.fn) != seen.end())
+        continue;
+      seen.push_back(pl.fn);
+      float ms = 0;
+      if (hipEventRecord(ev[0], st->stream) != hipSuccess ||
+          launch_update_as(st, st->cells[0], st->ncells[0], iter, 0, 0, 0, 1, nullptr) ||
+          hipEventRecord(ev[1], st->stream) != hipSuccess || harvest_events(st) ||
+          hipEventSynchronize(ev[1]) != hipSuccess ||
+          hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess || copy(false) != hipSuccess) {
+        rc = rc ? rc : fail("autotune: trial of kernel %d failed", k);
+        break;
+      }
+      if (best->ms < 0 || ms < best->ms) {
+        best->kernel = k;
+        best->views_per_lane = v;
+        best->ms = ms;
+      }
+    }
+  }
+  g_opt = saved;
+  if (hipStreamSynchronize(st->stream) != hipSuccess && !rc) rc = fail("autotune: sync");
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  release();
+  st->voxel_ms = vms;
+  st->voxel_launches = vl;
+  st->lds_launches = ll;
+  st->global_launches = gl;
+  memcpy(st->fam_launches, fam, sizeof(fam));
+  if (!rc && best->ms < 0) {
+    best->kernel = 0;  // nothing applied: the default choice
+    best->views_per_lane = 0;
+    best->ms = 0;
+  }
+  return rc;
+}
+
+static int autotune_pick(tb_stage *st, int iter, int *kernel, int *vpl) {
+  const TuneKey key = tune_key(st);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      *kernel = it->second.kernel;
+      *vpl = it->second.views_per_lane;
+      return 0;
+    }
+  }
+  tb_tune_entry e;
+  memset(&e, 0, sizeof(e));
+  memcpy(e.shape, key.v, sizeof(key.v));
+  if (autotune_run(st, iter, &e)) return -1;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = e;
+  }
+  *kernel = e.kernel;
+  *vpl = e.views_per_lane;
+  return 0;
+}
+
+extern "C" int tb_autotune(tb_stage *st, tb_tune_entry *out) {
+  int k = 0, v = 0;
+  if (autotune_pick(st, st->ord_iter > 0 ? st->ord_iter : 1, &k, &v)) return -1;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  if (out) *out = g_tune[tune_key(st)];
+  return 0;
+}
+
+extern "C" int tb_autotune_entries(tb_tune_entry *out, int32_t cap) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  int n = 0;
+  for (const auto &kv : g_tune) {
+    if (out && n < cap) out[n] = kv.second;
+    n++;
+  }
+  return n;
+}
+
+extern "C" int tb_autotune_put(const tb_tune_entry *e) {
+  if (e->kernel < 0 || e->kernel > 9 || e->views_per_lane < 0 || e->views_per_lane > 2)
+    return fail("tb_autotune_put: kernel %d / views per lane %d", e->kernel, e->views_per_lane);
+  TuneKey k;
+  memcpy(k.v, e->shape, sizeof(k.v));
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[k] = *e;
+  return 0;
+}
+
+extern "C" void tb_autotune_clear(void) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune.clear();
 }
 
 extern "C" int tb_get_stats(tb_stage *st, tb_stats *out) {
