@@ -65,7 +65,11 @@ typedef struct {
                                      applies (3- and 5-bin columns), 9 =
                                      the batched finest-stage kernel
                                      (3-bin columns, one detector row per
-                                     slice, up to 315 views per cell) */
+                                     slice, up to 315 views per cell), 10 =
+                                     the split-cell kernel (each cell's
+                                     views over several workgroups of one
+                                     cooperative launch; kernel 0 takes it
+                                     for colours of at most 32 cells) */
   int32_t views_per_lane;         /* views per theta-wave lane of the
                                      multi-wavefront LDS kernel: 0 = the
                                      fewest waves that keep one wavefront
@@ -95,6 +99,10 @@ typedef struct {
                                      configuration the autotuner measured
                                      fastest for the stage's shape, timing
                                      it on first use (default 0) */
+  int32_t split_segments;         /* workgroups per cell of the split-cell
+                                     kernel: 0 = as many as the compute
+                                     units allow (default), else at most
+                                     this many (2 to 64) */
 } tb_options;
 
 void tb_get_options(tb_options *opt);
@@ -227,6 +235,7 @@ enum {
   TB_K_MWD = 7,       /* k_icd_mwd: as k_icd_mwg, rows of the next voxels
                          in flight */
   TB_K_BAT = 8,       /* k_icd_bat: the finest stage in batches of voxels */
+  TB_K_SPLIT = 9,     /* k_icd_split: each cell's views over several workgroups */
   TB_KERNEL_FAMILIES = 12
 };
 typedef struct {

@@ -34,16 +34,17 @@ def pair(scan, stage_idx=0, mult_xy=1, mult_z=1, prev_mult_z=0, var_est=1.0, pre
     return o, g
 
 
-@pytest.fixture(params=[0, 1, 2, 3, 4, 5, 6, 7, 8, 9],
+@pytest.fixture(params=[0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10],
                 ids=["auto", "onevoxel", "onewave", "multiwave", "mwglobal", "wide", "wpass",
-                     "wloop", "mwdeep", "batched"])
+                     "wloop", "mwdeep", "batched", "split"])
 def kernel(request):
     """Voxel-update kernel: 0 = the fastest that applies, 1 = the one-voxel
     kernel everywhere, 2 = the one-wavefront LDS kernel, 3 = the
     multi-wavefront LDS kernel, 4 = the multi-wavefront kernel with e in
     global memory, 5 = the wide-column multi-wavefront kernels, 6 = the
     pass-based ones alone, 7 = the looped pass kernel alone, 8 = the
-    deep-pipeline global-e kernel, 9 = the batched finest-stage kernel."""
+    deep-pipeline global-e kernel, 9 = the batched finest-stage kernel, 10 =
+    the split-cell kernel."""
     old = tb.get_options().kernel
     tb.set_options(kernel=request.param)
     yield request.param
@@ -62,6 +63,7 @@ OPTION_FAMILIES = {
     7: {tb.K_WLOOP},
     8: {tb.K_MWD},
     9: {tb.K_BAT},
+    10: {tb.K_SPLIT},
 }
 
 

@@ -45,7 +45,8 @@ class TbOptions(C.Structure):
                 ("write_files", C.c_int32), ("order_seed", C.c_uint32), ("verbose", C.c_int32),
                 ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
                 ("init_fbp", C.c_int32), ("no_column_table", C.c_int32),
-                ("theta_fp", C.c_int32), ("row_lookahead", C.c_int32), ("autotune", C.c_int32)]
+                ("theta_fp", C.c_int32), ("row_lookahead", C.c_int32), ("autotune", C.c_int32),
+                ("split_segments", C.c_int32)]
 
 
 class TbStats(C.Structure):
@@ -73,7 +74,7 @@ class TbTuneEntry(C.Structure):
                 ("ms", C.c_float), ("reserved", C.c_int32)]
 
 # voxel-update kernel families (TB_K_* in timbir_dev.h)
-K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP, K_MWD, K_BAT = range(9)
+K_ONE_VOXEL, K_LDS, K_MW, K_MWG, K_WIDE, K_WPASS, K_WLOOP, K_MWD, K_BAT, K_SPLIT = range(10)
 
 
 class TimbirError(RuntimeError):

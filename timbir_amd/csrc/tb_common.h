@@ -134,4 +134,16 @@ TB_HD uint32_t perm(uint32_t n, uint32_t N, uint32_t key) {
 // rounded up to whole 16-byte words.
 TB_HD constexpr int col_rec(int nb) { return (nb + 4) / 4 * 4; }
 
+// Cross-workgroup exchange of the split-cell voxel kernel (k_icd_split):
+// per cell slot and voxel parity, one record per workgroup of its partial
+// theta sums, each 64-bit word tagged with the voxel's sequence number; a
+// word set when a wait timed out and one the host may set to end every wait.
+struct SplitBuf {
+  unsigned long long *rec;  // [slots][2][smax][4]: theta1 high, theta1 low, theta2, unused
+  unsigned *err;            // set when a wait timed out or was aborted
+  const unsigned *abort;    // nonzero: end every wait
+  int S, smax;
+  unsigned spin_limit;
+};
+
 }  // namespace tb

@@ -37,6 +37,9 @@ template <int NB, int MZ, int VPL, int MAXTH, int D, int C>
 __global__ void k_icd_mwd(KP, const int2 *, int, int, int, int, int);
 template <int MAXTH>
 __global__ void k_icd_bat(KP, const int2 *, int, int, int, int, int);
+template <int NBX, int R, int MAXTH>
+__global__ void k_icd_split(KP, const int2 *, int, int, int, int, int, SplitBuf);
+__global__ void k_split_reset(SplitBuf, int);
 template <int NB, int MZ, int VPL, int MAXTH>
 __global__ void k_icd_mw_m(const KP *, const int4 *, int, int, int, int);
 template <int NB, int MZ, int VPL, int MAXTH, int TF>
@@ -80,7 +83,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -131,6 +134,12 @@ struct tb_stage {
   float *dummy = nullptr;
   int *single = nullptr;
   float *colt = nullptr;  // column table of the multi-wavefront kernel (built on first use)
+  // the split-cell kernel's exchange buffers (allocated on first use) and
+  // the device's compute units
+  unsigned long long *split_rec = nullptr;  // [slots][2][64][4] tagged partials
+  unsigned *split_w = nullptr;              // err, abort
+  int split_slots = 0, n_cus = 0;
+  bool split_ran = false;
   int colt_nb = 0;
   size_t lds_max = 65536;
   int2 *cells[2] = {nullptr, nullptr};
@@ -256,7 +265,8 @@ extern "C" void tb_stage_destroy(tb_stage *st) {
                   st->mask, st->nh_list, st->nh_count, st->nh_ord, st->nh_ord_count, st->ord,
                   st->ord_off, st->dummy, st->single, st->colt, st->cells[0], st->cells[1],
                   st->cell_voxmag, st->cell_upd, st->cell_zero, st->cell_ent, st->off_pnum,
-                  st->off_pden, st->off_num, st->off_den, st->off_gamma, st->red};
+                  st->off_pden, st->off_num, st->off_den, st->off_gamma, st->red,
+                  st->split_rec, st->split_w};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (st->red_h) (void)hipHostFree(st->red_h);
@@ -726,6 +736,15 @@ static hipEvent_t *ev_pair(tb_stage *st) {
 }
 
 static int harvest_events(tb_stage *st) {
+  if (st->split_ran) {
+    // a cross-workgroup wait of the split-cell kernel that timed out
+    unsigned err = 0;
+    HIPCHK(hipMemcpyAsync(&err, st->split_w, sizeof(err),
+                          hipMemcpyDeviceToHost, st->stream));
+    HIPCHK(hipStreamSynchronize(st->stream));
+    st->split_ran = false;
+    if (err) return fail("k_icd_split: a cross-workgroup wait timed out (results invalid)");
+  }
   if (st->ev_used == 0) return 0;
   HIPCHK(hipEventSynchronize(st->ev[st->ev_used - 1]));
   for (int i = 0; i < st->ev_used; i += 2) {
@@ -969,6 +988,105 @@ static int ensure_columns(tb_stage *st, int nb) {
 
 static int autotune_pick(tb_stage *st, int iter, int *kernel, int *vpl);
 
+// The split-cell kernel (k_icd_split) for a colour of `ncells` cells: the
+// column width and rows per lane of the stage, S segments per cell (as many
+// as the compute units allow, each of at least 4 views, at most 64), a
+// cooperative launch so that every workgroup is resident.  Sets *launched;
+// returns 0 also when the kernel does not serve the stage.
+constexpr int kSplitMaxCells = 32;
+constexpr int kSplitSegMax = 64;
+static int launch_split(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk, int seq,
+                        int count_mag, bool *launched) {
+  const int w = 2 * st->mult_xy + 1;
+  const int nbx = w <= 3 ? 3 : w <= 5 ? 5 : w <= 9 ? 9 : w <= 17 ? 17 : 0;
+  const int mz = st->mult_z;
+  if (!nbx || st->N_r < nbx || (nbx == 3 && mz != 1) || ncells <= 0) return 0;
+  const int R = (nbx > 3 && mz % 2 == 0) ? 2 : 1;
+  const int ngr = mz / R;
+  using split_fn = void (*)(KP, const int2 *, int, int, int, int, int, SplitBuf);
+  split_fn fn = nbx == 17 ? (R == 2 ? k_icd_split<17, 2, 8> : k_icd_split<17, 1, 8>)
+              : nbx == 9  ? (R == 2 ? k_icd_split<9, 2, 8> : k_icd_split<9, 1, 8>)
+              : nbx == 5  ? (R == 2 ? k_icd_split<5, 2, 8> : k_icd_split<5, 1, 8>)
+                          : k_icd_split<3, 1, 8>;
+  if (!st->n_cus) {
+    int dev = 0, n = 0, coop = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    st->n_cus = coop ? n : -1;
+  }
+  if (st->n_cus <= 0) return 0;
+  // segments: views of at least 4 each, lanes of at most 8 theta waves
+  int S = std::min(kSplitSegMax, std::max(1, st->n_cus / ncells));
+  if (g_opt.split_segments > 0) S = std::min(S, g_opt.split_segments);
+  S = std::min(S, std::max(1, st->max_nv / 4));
+  int vseg = (st->max_nv + S - 1) / S;
+  while (vseg * ngr > 8 * 64 && S < kSplitSegMax) {
+    S++;
+    vseg = (st->max_nv + S - 1) / S;
+  }
+  if (vseg * ngr > 8 * 64) return 0;
+  int nth = (vseg * ngr + 63) / 64;
+  size_t lds = (size_t)vseg * kHrStride * sizeof(float);
+  if (lds + 4096 > st->lds_max) return 0;
+  HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // every workgroup resident at once
+  int per_cu = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * (nth + 1), lds));
+  while (S >= 2 && (long long)ncells * S > (long long)per_cu * st->n_cus) {
+    S--;
+    vseg = (st->max_nv + S - 1) / S;
+    if (vseg * ngr > 8 * 64) return 0;
+    const int nth2 = (vseg * ngr + 63) / 64;
+    const size_t lds2 = (size_t)vseg * kHrStride * sizeof(float);
+    if (nth2 != nth || lds2 != lds) {
+      nth = nth2;
+      lds = lds2;
+      if (lds + 4096 > st->lds_max) return 0;
+      HIPCHK(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * (nth + 1), lds));
+    }
+  }
+  if (S < 2) return 0;
+  // exchange buffers, one slot per cell of the larger colour
+  const int slots = std::max(st->ncells[0], st->ncells[1]);
+  if (st->split_slots < slots) {
+    if (st->split_rec) (void)hipFree(st->split_rec);
+    st->split_rec = nullptr;
+    st->split_slots = 0;
+    HIPCHK(dalloc(&st->split_rec, (size_t)slots * 2 * kSplitSegMax * 4));
+    if (!st->split_w) {
+      HIPCHK(dalloc(&st->split_w, 2));
+      hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, st->stream,
+                         reinterpret_cast<float *>(st->split_w), (size_t)2, 0.0f);
+    }
+    st->split_slots = slots;
+  }
+  if (ensure_columns(st, nbx) < 0) return -1;
+  KP P = st->kp();
+  if (seq == 0 && ensure_order(st, iter)) return -1;
+  SplitBuf B;
+  B.rec = st->split_rec;
+  B.err = st->split_w;
+  B.abort = st->split_w + 1;
+  B.S = S;
+  B.smax = kSplitSegMax;
+  B.spin_limit = 1u << 22;  // about a second per wait
+  hipLaunchKernelGGL(k_split_reset, dim3(std::min(64, st->split_slots)), dim3(256), 0, st->stream,
+                     B, st->split_slots);
+  HIPCHK(hipGetLastError());
+  void *args[] = {&P, &cells, &iter, &chunk, &seq, &count_mag, const_cast<int *>(&ngr), &B};
+  HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(fn), dim3(ncells * S),
+                                    dim3(64 * (nth + 1)), args, (unsigned)lds, st->stream));
+  st->global_launches++;
+  st->fam_launches[TB_K_SPLIT]++;
+  st->split_ran = true;
+  *launched = true;
+  return 0;
+}
+
 // The voxel-update kernel launch_update picks for a stage and phase, for
 // callers that launch it themselves (the multi-problem batch):
 //   kind 0: (KP, cells, iter, chunk, seq, count_mag, extra), block `block`
@@ -1206,6 +1324,15 @@ static int launch_update_as(tb_stage *st, const int2 *cells, int ncells, int ite
   if (!launched && tf != 0)
     return fail("theta_fp %d: FP32 theta sums exist in the multi-wavefront global-e kernel "
                 "with 3- or 5-bin columns only, which does not serve this stage", tf);
+  // few cells per colour (single-frame scans' coarse stages): the
+  // wide-column update with each cell's views split over S workgroups of one
+  // cooperative launch (kernel 10 asks for it alone; never in plan mode,
+  // whose callers may capture the launch in a graph)
+  if (!launched && !plan && tf == 0 &&
+      (g_opt.kernel == 10 || (g_opt.kernel == 0 && ncells <= kSplitMaxCells))) {
+    const int rc = launch_split(st, cells, ncells, iter, chunk, seq, count_mag, &launched);
+    if (rc < 0) return -1;
+  }
   // kernel 6 asks for the pass form alone
   const bool wpass_first = g_opt.kernel == 6;
   if (!launched && (g_opt.kernel == 0 || g_opt.kernel == 5) && 2 * st->mult_xy + 1 <= 17 &&
@@ -1940,7 +2067,7 @@ static int autotune_run(tb_stage *st, int iter, tb_tune_entry *best) {
   if (hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess ||
       copy(true) != hipSuccess)
     rc = fail("autotune: events or copies");
-  const int kernels[] = {3, 4, 5, 6, 7, 8, 9, 2};
+  const int kernels[] = {3, 4, 5, 6, 7, 8, 9, 10, 2};
   for (int k : kernels) {
     for (int v = (k == 3 ? 1 : 0); v <= (k == 3 ? 2 : 0) && !rc; v++) {
       g_opt = saved;
@@ -2033,7 +2160,7 @@ extern "C" int tb_autotune_entries(tb_tune_entry *out, int32_t cap) {
 }
 
 extern "C" int tb_autotune_put(const tb_tune_entry *e) {
-  if (e->kernel < 0 || e->kernel > 9 || e->views_per_lane < 0 || e->views_per_lane > 2)
+  if (e->kernel < 0 || e->kernel > 10 || e->views_per_lane < 0 || e->views_per_lane > 2)
     return fail("tb_autotune_put: kernel %d / views per lane %d", e->kernel, e->views_per_lane);
   TuneKey k;
   memcpy(k.v, e->shape, sizeof(k.v));

@@ -2439,6 +2439,343 @@ template __global__ void k_icd_wide<17, 2, 8>(KP, const int2 *, int, int, int, i
 template __global__ void k_icd_wide_m<17, 2, 8>(const KP *, const int4 *, int, int, int, int);
 
 // ---------------------------------------------------------------------------
+// k_icd_split: the wide-column voxel update with each cell's views split
+// over S workgroups, for stages whose colours hold few cells (single-frame
+// scans' coarse stages: 2 cells per colour at (8,8) leave 254 of 256 CUs
+// idle).  Grid: cells x S workgroups, launched cooperatively so that all are
+// resident.  Workgroup (cell, s) owns views [s nv / S, (s + 1) nv / S) of
+// the cell's frame, with k_icd_wide's lane layout over them; per voxel
+//   its theta waves reduce their partial theta sums in LDS (barrier 1); its
+//   last wave, which has evaluated the voxel's qGGMRF surrogate from the
+//   neighbours, adds them in a fixed tree and publishes the workgroup's
+//   partial as a record tagged with the voxel's sequence number; it then
+//   reads all S records (lane s polls workgroup s's until its tags match),
+//   adds them in a fixed tree over the lanes, solves, stores the voxel and
+//   hands the step to its theta waves in LDS (barrier 2), which update
+//   their entries.
+// Every workgroup of the cell solves every voxel from the same S partials in
+// the same order and the same neighbour values, so all find the same step
+// bit for bit and all store the same voxel value; the neighbours a
+// workgroup reads are the values it stored itself earlier in the launch, or
+// values from before it (no other workgroup can have stored a voxel the
+// reader has not: each stores voxel n only after every workgroup has
+// published its partial of voxel n, which follows its reading of voxel n's
+// neighbours).  Zero-skipped voxels are decided from those values alike and
+// exchange nothing.
+// Every wait is bounded (spin_limit polls) and also ends when the host's
+// abort word is nonzero: then the workgroup records the timeout in err, takes
+// dx = 0 for the rest of the sequence and stops waiting, so every wave still
+// reaches the end.  Records are double-buffered by voxel parity (a workgroup
+// publishes voxel n + 2's record only after every workgroup has published
+// voxel n + 1's, which each does after it has read voxel n's) and zeroed
+// before each launch.
+// ---------------------------------------------------------------------------
+// The records are written and read only with device-scope atomic 64-bit
+// accesses, which go to the device's coherence point past the XCDs' L2s;
+// a word carries its own tag, so no ordering between words and no cache-wide
+// fence (an L2 invalidate or write-back per poll) is needed.
+__global__ void k_split_reset(SplitBuf B, int slots) {
+  const size_t n = (size_t)slots * 2 * B.smax * 4;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    B.rec[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) B.err[0] = 0u;
+}
+
+template <int NBX, int R, int MAXTH>
+__global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2 *__restrict__ cells,
+                                                             int iter, int chunk, int seq,
+                                                             int count_mag, int ngr, SplitBuf B) {
+  __shared__ StepIn S;
+  __shared__ float s_dx[2];
+  extern __shared__ float s_H[];  // [views of the segment][kHrStride]
+  const int slot = blockIdx.x / B.S, seg = blockIdx.x - slot * B.S;
+  const int2 cl = cells[slot];
+  const int f = cl.x, z = cl.y;
+  const int cell = f * P.N_z + z;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nth = (int)(blockDim.x >> 6) - 1;
+  const int vf = P.view_off[f], nvf = P.view_off[f + 1] - vf;
+  const int qs = (int)((long long)nvf * seg / B.S), qe = (int)((long long)nvf * (seg + 1) / B.S);
+  const int vb = vf + qs, nv = qe - qs;
+
+  const int *list;
+  int n0 = 0, cnt = 0;
+  if (seq == 0) {
+    list = P.ord + (size_t)cell * P.ord_cap;
+    n0 = P.ord_off[(size_t)cell * (P.K + 1) + chunk];
+    cnt = P.ord_off[(size_t)cell * (P.K + 1) + chunk + 1] - n0;
+  } else if (seq == 1) {
+    list = P.nh_ord + (size_t)cell * P.nh_cap;
+    cnt = P.nh_ord_count[cell];
+  } else {
+    list = P.single;
+    cnt = 1;
+  }
+  if (cnt <= 0) return;  // the same for every workgroup of the cell
+  for (int i = threadIdx.x; i < nv * kHrStride; i += blockDim.x)
+    s_H[i] = P.hr[(size_t)P.view_idx[vb + i / kHrStride] * kHrStride + i % kHrStride];
+  if (threadIdx.x < 32) {
+    (&S.t1[0][0])[threadIdx.x] = 0.0;
+    (&S.t2[0][0])[threadIdx.x] = 0.0f;
+  }
+  __syncthreads();
+  const int *lst = list + n0;
+
+  if (wave < nth) {
+    // ---------------- theta wave (k_icd_wide's, over the segment) ----------------
+    const int t = wave * 64 + lane;
+    const int q = t / ngr, g = t - q * ngr;
+    const bool act = q < nv;
+    const int v = P.view_idx[nv > 0 ? vb + (act ? q : 0) : vf];  // a segment may hold no view
+    double Au, Bu, Cu;
+    col_coef(P, v, Au, Bu, Cu);
+    const float *H = s_H + (act ? q : 0) * kHrStride;
+    const float *eb = P.e + sidx(P, z, v, 0, g * R);
+    const float *wb = P.w + sidx(P, z, v, 0, g * R);
+    float *ebw = P.e + sidx(P, z, v, 0, g * R);
+    unsigned long long nent = 0;
+    int pix_next = lst[0];
+    for (int n = 0; n < cnt; n++) {
+      const int par = n & 1;
+      const int pix = pix_next;
+      pix_next = lst[min(n + 1, cnt - 1)];
+      const int yy = pix >> 16, xx = pix & 0xffff;
+      const double u = col_u(xx, yy, Au, Bu, Cu);
+      int loc, hic;
+      const int w0 = act ? col_window(P, u, NBX, loc, hic) : -1;
+      const int ws = max(w0, 0);
+      float e[NBX][R], w[NBX][R], a[NBX];
+#pragma unroll
+      for (int j = 0; j < NBX; j++) {
+        ld_r<R>(eb + (size_t)(ws + j) * P.mz, e[j]);
+        ld_r<R>(wb + (size_t)(ws + j) * P.mz, w[j]);
+      }
+      if (P.colt && P.colt_rec == col_rec(NBX)) {
+        constexpr int REC = col_rec(NBX);
+        const float4 *r4 = reinterpret_cast<const float4 *>(
+            P.colt + (((size_t)f * P.N_x * P.N_y + (size_t)yy * P.N_x + xx) * P.colt_nv +
+                      (act ? qs + q : 0)) * REC);
+        float rr[REC];
+#pragma unroll
+        for (int k = 0; k < REC / 4; k++) {
+          const float4 x4 = r4[k];
+          rr[4 * k] = x4.x;
+          rr[4 * k + 1] = x4.y;
+          rr[4 * k + 2] = x4.z;
+          rr[4 * k + 3] = x4.w;
+        }
+#pragma unroll
+        for (int j = 0; j < NBX; j++) a[j] = w0 >= 0 ? rr[j] : 0.0f;
+      } else {
+#pragma unroll
+        for (int j = 0; j < NBX; j++) a[j] = w0 >= 0 ? col_value(P, H, u, ws + j, loc, hic) : 0.0f;
+      }
+      double t1 = 0.0;
+      float t2 = 0.0f;
+#pragma unroll
+      for (int j = 0; j < NBX; j++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const float ee = e[j][r], ww = w[j][r];
+          const float cw = huber_sel(P, ee, ww)
+                               ? ww
+                               : P.Td * __builtin_amdgcn_sqrtf(ww) * __builtin_amdgcn_rcpf(fabsf(ee));
+          const float ac = a[j] * cw;
+          t1 = t1 + (double)(-ee * ac);
+          t2 = t2 + a[j] * ac;
+        }
+        nent += a[j] > 0.0f ? R : 0;
+      }
+      wave_total_df(t1, t2);
+      if (lane == 0) {
+        S.t1[par][wave] = t1;
+        S.t2[par][wave] = t2;
+      }
+      __syncthreads();  // 1: the workgroup's partial sums are in LDS
+      __syncthreads();  // 2: the voxel's step is in LDS
+      const float dx = s_dx[par];
+      if (dx != 0.0f) {
+#pragma unroll
+        for (int j = 0; j < NBX; j++) {
+          if (a[j] != 0.0f) {
+            float o[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) o[r] = fmaf(-a[j], dx, e[j][r]);
+            if constexpr (R == 1) ebw[(size_t)(ws + j) * P.mz] = o[0];
+            else *reinterpret_cast<float2 *>(ebw + (size_t)(ws + j) * P.mz) = make_float2(o[0], o[1]);
+          }
+        }
+      }
+    }
+    const double nsum = wave_sum_d((double)nent);
+    if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
+  } else {
+    // ---------------- exchange wave: the prior, the exchange, the solve ----------------
+    const bool lead = seg == 0;
+    const int ln = lane;
+    PriorLane C;
+    C.ln = ln;
+    int ndz = 0, ndy = 0, ndx = 0, ndt = 0;
+    C.ng = 0.0f;
+    C.nsq = 1.0f;
+    C.nisqp = 1.0f;
+    C.nc = 1.0f;
+    if (ln < 29) {
+      if (ln < 27) {
+        ndz = ln / 9 - 1;
+        ndy = (ln / 3) % 3 - 1;
+        ndx = ln % 3 - 1;
+        C.ng = P.sfilt[ln];
+        C.nsq = P.sSq, C.nisqp = 1.0f / P.sSqp, C.nc = P.c_s;
+      } else {
+        ndt = ln == 27 ? -1 : 1;
+        C.ng = P.tfilt;
+        C.nsq = P.sTq, C.nisqp = 1.0f / P.sTqp, C.nc = P.c_t;
+      }
+    }
+    C.two_c = 2.0f * C.nc;
+    const int zz = z + ndz, ff = f + ndt;
+    const bool zok = ln < 29 && zz >= 0 && zz < P.N_z && ff >= 0 && ff < P.N_time;
+    const int zc = min(max(zz, 0), P.N_z - 1), fc = min(max(ff, 0), P.N_time - 1);
+    const int nplane = (fc * P.N_z + zc) * P.N_x * P.N_y;
+    const int cbase = cell * P.N_x * P.N_y;
+    auto nb_load = [&](int pix) -> float {
+      const int yy = pix >> 16, xx = pix & 0xffff;
+      const int jj = yy + ndy, kk = xx + ndx;
+      const bool nok = zok & (jj >= 0) & (jj < P.N_y) & (kk >= 0) & (kk < P.N_x);
+      return P.obj[(unsigned)(nok ? nplane + jj * P.N_x + kk : P.nobj + 64)];
+    };
+    unsigned long long *rec = B.rec + (size_t)slot * 2 * B.smax * 4;
+    bool dead = false;
+    double vmacc = 0.0;
+    unsigned long long nupd = 0, nzero = 0;
+    float nbv = nb_load(lst[0]);
+    for (int n = 0; n < cnt; n++) {
+      const int par = n & 1;
+      const int pix = lst[n];
+      // the voxel's surrogate (XT_ICD_update.c:134-183) and zero-skip
+      // (XT_VoxUpdate.c:236-259), from neighbour values read after the
+      // previous voxel's store
+      const float V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
+      const bool anypos = (__ballot((ln != 13) & (ln < 29) & (nbv > 0.0f)) != 0ull);
+      float qb, qbx;
+      prior_pair(C, nbv, V, qb, qbx);
+      const float pb = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), 0));
+      const float pbx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qbx), 0));
+      const bool skip = (V == 0.0f) && iter > 1 && !anypos;
+      __syncthreads();  // 1
+      float xn = V, dx = 0.0f;
+      if (!skip) {
+        // the workgroup's partial: a fixed tree over the MAXTH wave slots
+        double a1[MAXTH];
+        float a2[MAXTH];
+#pragma unroll
+        for (int w = 0; w < MAXTH; w++) {
+          a1[w] = S.t1[par][w];
+          a2[w] = S.t2[par][w];
+        }
+#pragma unroll
+        for (int m = MAXTH; m > 1; m = (m + 1) / 2) {
+          const int h = (m + 1) / 2;
+#pragma unroll
+          for (int w = 0; w + h < m; w++) {
+            a1[w] = a1[w] + a1[w + h];
+            a2[w] = a2[w] + a2[w + h];
+          }
+        }
+        // publish: theta1 as a high and a low float (48 significant bits),
+        // theta2, each word tagged n + 1
+        const unsigned long long tag = (unsigned long long)(unsigned)(n + 1) << 32;
+        unsigned long long *rp = rec + (size_t)par * B.smax * 4;
+        if (lane < 3) {
+          const float hi = (float)a1[0];
+          const float lo = (float)(a1[0] - (double)hi);
+          const float pay = lane == 0 ? hi : lane == 1 ? lo : a2[0];
+          __hip_atomic_store(rp + seg * 4 + lane, tag | __float_as_uint(pay), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // every workgroup's record: lane s polls workgroup s's
+        unsigned long long r0 = 0, r1 = 0, r2 = 0;
+        bool have = lane >= B.S;
+        for (unsigned k = 0; !dead; k++) {
+          if (!have) {
+            const unsigned long long *q = rp + lane * 4;
+            r0 = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            r1 = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            r2 = __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            have = ((r0 & r1 & r2) >> 32) == (tag >> 32) && ((r0 | r1 | r2) >> 32) == (tag >> 32);
+          }
+          if (__ballot(!have) == 0ull) break;
+          const unsigned ab = __builtin_amdgcn_readfirstlane(
+              __hip_atomic_load(B.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+          if (k >= B.spin_limit || ab != 0u) dead = true;
+          else __builtin_amdgcn_s_sleep(1);
+        }
+        double p1 = 0.0;
+        float p2 = 0.0f;
+        if (!dead && lane < B.S) {
+          p1 = (double)__uint_as_float((unsigned)r0) + (double)__uint_as_float((unsigned)r1);
+          p2 = __uint_as_float((unsigned)r2);
+        }
+        // the S partials in a fixed tree over the lanes: the same sums, in
+        // the same order, in every workgroup
+        wave_total_df(p1, p2);
+        if (!dead) {
+#pragma clang fp contract(off)
+          const float th1 = (float)p1;
+          const float num = P.alpha * (pbx - th1);
+          const float step = num * __builtin_amdgcn_rcpf(pb + p2);
+          xn = V + step;
+          dx = xn - V;
+        }
+      }
+      // the object and update map (the same values from every workgroup),
+      // then the step for the theta waves
+      if (lane == 0) {
+        const int lin = cbase + (pix >> 16) * P.N_x + (pix & 0xffff);
+        P.obj[lin] = xn;
+        P.mag[lin] = skip ? 0.0f : fabsf(dx);
+        s_dx[par] = dx;
+      }
+      if (lead) {
+        vmacc += (lane == 0 && count_mag && !skip) ? fabs((double)xn) : 0.0;
+        nupd += (lane == 0 && !skip) ? 1ull : 0ull;
+        nzero += (lane == 0 && skip) ? 1ull : 0ull;
+      }
+      // the next voxel's neighbours, read after this voxel's store
+      if (n + 1 < cnt) nbv = nb_load(lst[n + 1]);
+      __syncthreads();  // 2
+    }
+    if (dead && lane == 0) atomicOr(B.err, 1u);
+    const double vm = wave_sum_d(vmacc);
+    const double nu = wave_sum_d((double)nupd), nz0 = wave_sum_d((double)nzero);
+    if (lead && lane == 0 && seq != 2) {
+      P.cell_voxmag[cell] += vm;
+      P.cell_upd[cell] += (unsigned long long)nu;
+      P.cell_zero[cell] += (unsigned long long)nz0;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && seq != 2) {
+    unsigned long long ne = 0;
+    for (int w = 0; w < nth; w++) ne += S.nent[w];
+    atomicAdd(P.cell_ent + cell, ne);
+  }
+}
+
+
+// the split-cell kernel: column widths 17, 9, 5 (two rows per lane) and 3
+// (single-row slices), up to 8 theta waves per segment
+template __global__ void k_icd_split<17, 2, 8>(KP, const int2 *, int, int, int, int, int, SplitBuf);
+template __global__ void k_icd_split<17, 1, 8>(KP, const int2 *, int, int, int, int, int, SplitBuf);
+template __global__ void k_icd_split<9, 2, 8>(KP, const int2 *, int, int, int, int, int, SplitBuf);
+template __global__ void k_icd_split<9, 1, 8>(KP, const int2 *, int, int, int, int, int, SplitBuf);
+template __global__ void k_icd_split<5, 2, 8>(KP, const int2 *, int, int, int, int, int, SplitBuf);
+template __global__ void k_icd_split<5, 1, 8>(KP, const int2 *, int, int, int, int, int, SplitBuf);
+template __global__ void k_icd_split<3, 1, 8>(KP, const int2 *, int, int, int, int, int, SplitBuf);
+
+// ---------------------------------------------------------------------------
 // Multi-wavefront voxel update for wide columns and many views: the coarse
 // multires stages whose cells neither the column-table kernels nor
 // k_icd_wide can hold (hundreds of views per cell).  Theta wave w owns a
