@@ -101,8 +101,10 @@ typedef struct {
                                      it on first use (default 0) */
   int32_t split_segments;         /* workgroups per cell of the split-cell
                                      kernel: 0 = as many as the compute
-                                     units allow (default), else at most
-                                     this many (2 to 64) */
+                                     units allow (default), 1 = never use
+                                     it (kernel 0 takes the other
+                                     families), else at most this many
+                                     (2 to 64) */
 } tb_options;
 
 void tb_get_options(tb_options *opt);

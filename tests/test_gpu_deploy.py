@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 # reconstruct()'s schedule for 512 columns x 32 rows (XT_Engine.c:111-141)
 # and the kernel family that serves each stage
-STAGES = [((8, 8), tb.K_WIDE), ((4, 4), tb.K_MWG), ((2, 2), tb.K_MWG), ((1, 1), tb.K_MWG)]
+STAGES = [((8, 8), tb.K_SPLIT), ((4, 4), tb.K_MWG), ((2, 2), tb.K_MWG), ((1, 1), tb.K_MWG)]
 
 
 def log(msg):

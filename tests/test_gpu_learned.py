@@ -54,17 +54,21 @@ def runs():
         scan = datagen.interlaced4d(N=256, nz=8, n_frames=16, n_views=2048, seed=20269999)
         net = UNet(params=UNet.load_params(WEIGHTS))
         res = {"zero": converge.run(scan, "zero", tol=1e-4, cap=CAP)}
+        log("  zero start done")
         level = res["zero"]["rel_change"]["cost"] if res["zero"]["rel_change"] else None
         for start in ("fbp", "learned"):
             res[start] = converge.run(scan, start, tol=1e-4, cap=CAP, net=net,
                                       until=("rel_change", "target") if level else
                                       ("rel_change",), target=level)
+            log(f"  {start} start done")
+        def at(m):
+            return f"{m['iterations']} iterations ({m['seconds']:.2f} s, cost {m['cost']:.8e})" \
+                if m else f"not within {CAP} iterations"
+
         for start, r in res.items():
-            rc, tg = r["rel_change"], r["target"]
-            log(f"  {start:7s}: 1e-4 change after {rc and rc['iterations']} finest iterations "
-                f"({rc and rc['seconds']:.2f} s, cost {rc and rc['cost']:.8e}); zero start's "
-                f"level after {tg and tg['iterations']} ({tg and tg['seconds']:.2f} s); cost "
-                f"after iteration 1 {r['finest_cost_at'].get('1'):.8e}, last {r['final_cost']:.8e}")
+            log(f"  {start:7s}: 1e-4 change after {at(r['rel_change'])}; zero start's level "
+                f"after {at(r['target'])}; cost after iteration 1 "
+                f"{r['finest_cost_at'].get('1')}, last {r['final_cost']:.8e}")
         yield res
     finally:
         tb.set_options(**saved)

@@ -53,7 +53,7 @@ def kernel(request):
 
 # kernel option -> the kernel families it may run (TB_K_* in timbir_dev.h)
 OPTION_FAMILIES = {
-    0: {tb.K_LDS, tb.K_MW, tb.K_MWG, tb.K_WIDE, tb.K_WPASS, tb.K_WLOOP},
+    0: {tb.K_LDS, tb.K_MW, tb.K_MWG, tb.K_WIDE, tb.K_WPASS, tb.K_WLOOP, tb.K_SPLIT},
     1: {tb.K_ONE_VOXEL},
     2: {tb.K_LDS},
     3: {tb.K_MW},

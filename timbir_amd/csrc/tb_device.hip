@@ -995,12 +995,21 @@ static int autotune_pick(tb_stage *st, int iter, int *kernel, int *vpl);
 // returns 0 also when the kernel does not serve the stage.
 constexpr int kSplitMaxCells = 32;
 constexpr int kSplitSegMax = 64;
+// Split-cell launches on a device, from any stage and stream, run one after
+// another: each waits for the previous one's completion event, so two
+// cooperative grids are never resident together and the launch path's
+// set-up calls are made by one host thread at a time.  The events live as
+// long as the process.
+static std::mutex g_split_mu;
+static hipEvent_t g_split_ev[kMaxDevices];
 static int launch_split(tb_stage *st, const int2 *cells, int ncells, int iter, int chunk, int seq,
                         int count_mag, bool *launched) {
   const int w = 2 * st->mult_xy + 1;
   const int nbx = w <= 3 ? 3 : w <= 5 ? 5 : w <= 9 ? 9 : w <= 17 ? 17 : 0;
   const int mz = st->mult_z;
   if (!nbx || st->N_r < nbx || (nbx == 3 && mz != 1) || ncells <= 0) return 0;
+  if (g_opt.split_segments == 1) return 0;
+  std::lock_guard<std::mutex> guard(g_split_mu);
   const int R = (nbx > 3 && mz % 2 == 0) ? 2 : 1;
   const int ngr = mz / R;
   using split_fn = void (*)(KP, const int2 *, int, int, int, int, int, SplitBuf);
@@ -1074,12 +1083,18 @@ static int launch_split(tb_stage *st, const int2 *cells, int ncells, int iter, i
   B.S = S;
   B.smax = kSplitSegMax;
   B.spin_limit = 1u << 22;  // about a second per wait
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail("launch_split: device %d", dev);
+  if (g_split_ev[dev]) HIPCHK(hipStreamWaitEvent(st->stream, g_split_ev[dev], 0));
+  else HIPCHK(hipEventCreateWithFlags(&g_split_ev[dev], hipEventDisableTiming));
   hipLaunchKernelGGL(k_split_reset, dim3(std::min(64, st->split_slots)), dim3(256), 0, st->stream,
                      B, st->split_slots);
   HIPCHK(hipGetLastError());
   void *args[] = {&P, &cells, &iter, &chunk, &seq, &count_mag, const_cast<int *>(&ngr), &B};
   HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(fn), dim3(ncells * S),
                                     dim3(64 * (nth + 1)), args, (unsigned)lds, st->stream));
+  HIPCHK(hipEventRecord(g_split_ev[dev], st->stream));
   st->global_launches++;
   st->fam_launches[TB_K_SPLIT]++;
   st->split_ran = true;

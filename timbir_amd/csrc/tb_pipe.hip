@@ -2524,6 +2524,12 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
 
   if (wave < nth) {
     // ---------------- theta wave (k_icd_wide's, over the segment) ----------------
+    // Each lane keeps one view and one row group for the whole sequence, so
+    // the entries voxel n updates that voxel n + 1 also reads are this lane's
+    // own: with PF, voxel n + 1's rows are loaded while voxel n's exchange
+    // runs and those entries are then set to the values this lane stores,
+    // which is what a load after the store would return.
+    constexpr bool PF = NBX <= 9;
     const int t = wave * 64 + lane;
     const int q = t / ngr, g = t - q * ngr;
     const bool act = q < nv;
@@ -2535,17 +2541,14 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
     const float *wb = P.w + sidx(P, z, v, 0, g * R);
     float *ebw = P.e + sidx(P, z, v, 0, g * R);
     unsigned long long nent = 0;
-    int pix_next = lst[0];
-    for (int n = 0; n < cnt; n++) {
-      const int par = n & 1;
-      const int pix = pix_next;
-      pix_next = lst[min(n + 1, cnt - 1)];
+    // the window, (e, w) rows and A values of the voxel at pix
+    auto load_rows = [&](int pix, int &ws, float (&e)[NBX][R], float (&w)[NBX][R],
+                         float (&a)[NBX]) {
       const int yy = pix >> 16, xx = pix & 0xffff;
       const double u = col_u(xx, yy, Au, Bu, Cu);
       int loc, hic;
       const int w0 = act ? col_window(P, u, NBX, loc, hic) : -1;
-      const int ws = max(w0, 0);
-      float e[NBX][R], w[NBX][R], a[NBX];
+      ws = max(w0, 0);
 #pragma unroll
       for (int j = 0; j < NBX; j++) {
         ld_r<R>(eb + (size_t)(ws + j) * P.mz, e[j]);
@@ -2571,21 +2574,29 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
 #pragma unroll
         for (int j = 0; j < NBX; j++) a[j] = w0 >= 0 ? col_value(P, H, u, ws + j, loc, hic) : 0.0f;
       }
+    };
+    float ec[NBX][R], wc[NBX][R], ac[NBX];
+    int wsc;
+    load_rows(lst[0], wsc, ec, wc, ac);
+    int pix_next = lst[min(1, cnt - 1)];
+    for (int n = 0; n < cnt; n++) {
+      const int par = n & 1;
+      const bool more = n + 1 < cnt;
       double t1 = 0.0;
       float t2 = 0.0f;
 #pragma unroll
       for (int j = 0; j < NBX; j++) {
 #pragma unroll
         for (int r = 0; r < R; r++) {
-          const float ee = e[j][r], ww = w[j][r];
+          const float ee = ec[j][r], ww = wc[j][r];
           const float cw = huber_sel(P, ee, ww)
                                ? ww
                                : P.Td * __builtin_amdgcn_sqrtf(ww) * __builtin_amdgcn_rcpf(fabsf(ee));
-          const float ac = a[j] * cw;
-          t1 = t1 + (double)(-ee * ac);
-          t2 = t2 + a[j] * ac;
+          const float ac_ = ac[j] * cw;
+          t1 = t1 + (double)(-ee * ac_);
+          t2 = t2 + ac[j] * ac_;
         }
-        nent += a[j] > 0.0f ? R : 0;
+        nent += ac[j] > 0.0f ? R : 0;
       }
       wave_total_df(t1, t2);
       if (lane == 0) {
@@ -2593,18 +2604,59 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
         S.t2[par][wave] = t2;
       }
       __syncthreads();  // 1: the workgroup's partial sums are in LDS
+      float en[NBX][R], wn[NBX][R], an[NBX];
+      int wsn = 0;
+      const int pix1 = pix_next;
+      if constexpr (PF) {
+        if (more) {
+          load_rows(pix1, wsn, en, wn, an);
+          pix_next = lst[min(n + 2, cnt - 1)];
+        }
+      }
       __syncthreads();  // 2: the voxel's step is in LDS
       const float dx = s_dx[par];
       if (dx != 0.0f) {
+        float o[NBX][R];
+#pragma unroll
+        for (int j = 0; j < NBX; j++)
+#pragma unroll
+          for (int r = 0; r < R; r++) o[j][r] = fmaf(-ac[j], dx, ec[j][r]);
 #pragma unroll
         for (int j = 0; j < NBX; j++) {
-          if (a[j] != 0.0f) {
-            float o[R];
-#pragma unroll
-            for (int r = 0; r < R; r++) o[r] = fmaf(-a[j], dx, e[j][r]);
-            if constexpr (R == 1) ebw[(size_t)(ws + j) * P.mz] = o[0];
-            else *reinterpret_cast<float2 *>(ebw + (size_t)(ws + j) * P.mz) = make_float2(o[0], o[1]);
+          if (ac[j] != 0.0f) {
+            if constexpr (R == 1) ebw[(size_t)(wsc + j) * P.mz] = o[j][0];
+            else *reinterpret_cast<float2 *>(ebw + (size_t)(wsc + j) * P.mz) = make_float2(o[j][0], o[j][1]);
           }
+        }
+        if constexpr (PF) {
+          // entry ws' + j' of the next window is entry wsc + j of this one
+          // when j - j' = ws' - wsc
+          const int d = wsn - wsc;
+          if (more && __ballot(d > -NBX && d < NBX) != 0ull) {
+#pragma unroll
+            for (int jp = 0; jp < NBX; jp++)
+#pragma unroll
+              for (int j = 0; j < NBX; j++) {
+                const bool m = (d == j - jp) && ac[j] != 0.0f;
+#pragma unroll
+                for (int r = 0; r < R; r++) en[jp][r] = m ? o[j][r] : en[jp][r];
+              }
+          }
+        }
+      }
+      if (!more) break;
+      if constexpr (!PF) {
+        load_rows(pix1, wsn, en, wn, an);
+        pix_next = lst[min(n + 2, cnt - 1)];
+      }
+      wsc = wsn;
+#pragma unroll
+      for (int j = 0; j < NBX; j++) {
+        ac[j] = an[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          ec[j][r] = en[j][r];
+          wc[j][r] = wn[j][r];
         }
       }
     }
@@ -2651,12 +2703,13 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
     double vmacc = 0.0;
     unsigned long long nupd = 0, nzero = 0;
     float nbv = nb_load(lst[0]);
+    int pix = lst[0], pix1 = lst[min(1, cnt - 1)];
     for (int n = 0; n < cnt; n++) {
       const int par = n & 1;
-      const int pix = lst[n];
+      const bool more = n + 1 < cnt;
       // the voxel's surrogate (XT_ICD_update.c:134-183) and zero-skip
-      // (XT_VoxUpdate.c:236-259), from neighbour values read after the
-      // previous voxel's store
+      // (XT_VoxUpdate.c:236-259), from the neighbour values as they are
+      // after the previous voxel's store
       const float V = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(nbv), 13));
       const bool anypos = (__ballot((ln != 13) & (ln < 29) & (nbv > 0.0f)) != 0ull);
       float qb, qbx;
@@ -2665,6 +2718,14 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
       const float pbx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qbx), 0));
       const bool skip = (V == 0.0f) && iter > 1 && !anypos;
       __syncthreads();  // 1
+      // the next voxel's neighbours, loaded while this voxel is exchanged
+      // (this voxel, if it is one of them, is set after its solve)
+      float nbn = 0.0f;
+      int pix2 = pix1;
+      if (more) {
+        nbn = nb_load(pix1);
+        pix2 = lst[min(n + 2, cnt - 1)];
+      }
       float xn = V, dx = 0.0f;
       if (!skip) {
         // the workgroup's partial: a fixed tree over the MAXTH wave slots
@@ -2743,9 +2804,13 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
         nupd += (lane == 0 && !skip) ? 1ull : 0ull;
         nzero += (lane == 0 && skip) ? 1ull : 0ull;
       }
-      // the next voxel's neighbours, read after this voxel's store
-      if (n + 1 < cnt) nbv = nb_load(lst[n + 1]);
+      if (more && ln >= 9 && ln < 18 && (pix1 >> 16) + ndy == (pix >> 16) &&
+          (pix1 & 0xffff) + ndx == (pix & 0xffff))
+        nbn = xn;  // an in-plane neighbour: the value just stored
       __syncthreads();  // 2
+      nbv = nbn;
+      pix = pix1;
+      pix1 = pix2;
     }
     if (dead && lane == 0) atomicOr(B.err, 1u);
     const double vm = wave_sum_d(vmacc);
