@@ -156,6 +156,9 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--profile-steps", action="store_true",
                     help="only the timed steps (for counter and trace runs)")
+    ap.add_argument("--stop-rule", action="store_true",
+                    help="also time reconstruct()'s schedule on the workload from zero to the "
+                         "reference's stop rule at every stage")
     ap.add_argument("--kernel", type=int, default=0,
                     help="voxel-update kernel (tb_options.kernel; 0: the fastest that applies)")
     ap.add_argument("--row-lookahead", type=int, default=0,
@@ -266,6 +269,13 @@ def main():
                     sp[crit] = a["seconds"] / b["seconds"]
             ttc["learned_speedup_vs_zero"] = sp
         line["time_to_converged"] = ttc
+    if args.stop_rule:
+        from timbir_amd import batch
+        log("reference schedule to its stop rule ...")
+        stages = []
+        batch.reconstruct_stages(scan, timing=stages)
+        line["time_to_stop_rule"] = {"seconds": sum(x["seconds"] for x in stages),
+                                     "stages": stages}
     if not args.no_cpu and not args.small:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(scan, stage_kw, args.cpu_threads or cpu_threads())

@@ -46,11 +46,15 @@ def multires_schedule(proj_rows: int, proj_cols: int) -> list:
     return list(zip(mxy[::-1], mz[::-1]))
 
 
-def reconstruct_stages(scan: dict) -> np.ndarray:
+def reconstruct_stages(scan: dict, timing: list | None = None) -> np.ndarray:
     """reconstruct()'s schedule through the stage API, without files: each
     stage from the previous one in HBM (XT_Engine.c:156-182), run to the
     reference's stop rule.  Returns the finest object
-    [recon_num][rows][cols][cols]."""
+    [recon_num][rows][cols][cols].  timing, when given, receives one dict
+    per stage: mult, ICD iterations, seconds (set-up and hand-over included,
+    the device synchronized at the end of the stage) and the kernel
+    families' launch counts."""
+    import time
     import timbir_amd as tb
     sched = multires_schedule(scan["proj_rows"], scan["proj_cols"])
     prev = None
@@ -58,6 +62,7 @@ def reconstruct_stages(scan: dict) -> np.ndarray:
     st = None
     try:
         for s, (mxy, mz) in enumerate(sched):
+            t0 = time.perf_counter()
             st = tb.Stage(scan, mult_xy=mxy, mult_z=mz, stage_idx=s,
                           prev_mult_z=sched[s - 1][1] if s else 0, var_est=var_est)
             if prev is not None:
@@ -65,7 +70,12 @@ def reconstruct_stages(scan: dict) -> np.ndarray:
                 st.set_offset(prev.offset())
                 prev.close()
                 prev = None
-            st.stage_run(0, with_cost=False)
+            costs = st.stage_run(0, with_cost=False)
+            if timing is not None:
+                st.synchronize()
+                timing.append({"mult": [mxy, mz], "iterations": len(costs) - 1,
+                               "seconds": time.perf_counter() - t0,
+                               "family_launches": st.stats()["family_launches"]})
             if s < len(sched) - 1:
                 prev, var_est, st = st, st.var_est, None
         return st.object()
