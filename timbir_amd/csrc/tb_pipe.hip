@@ -1123,6 +1123,34 @@ __device__ __forceinline__ void k_icd_mwg_body(const KP &P, const int2 cl, int i
       const int lin = pix != kNoPix ? (pix >> 16) * P.N_x + (pix & 0xffff) : 0;
       return ((fbase + lin) * (size_t)P.colt_nv + qv[k]) * REC;
     };
+    // The shuffled (HICD) voxels of 3-bin columns, one view per lane, take
+    // their A value and window from the view's geometry in the lane, with
+    // the column table's own arithmetic (the same bits): their records
+    // would be 2 KB reads at random places of the table.  Raster-ordered
+    // lines read the table, and so do lanes of several views, for which the
+    // arithmetic costs more than the reads.
+    const bool calc = NB == 3 && VPL == 1 && seq == 0;
+    double cAu[VPL], cBu[VPL], cCu[VPL];
+    const float *cH[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; k++) {
+      const int v = nv > 0 ? P.view_idx[vb + qv[k]] : 0;
+      col_coef(P, v, cAu[k], cBu[k], cCu[k]);
+      cH[k] = P.hr + (size_t)v * kHrStride;
+    }
+    auto col_at = [&](int pix, int k, float &a, int &w0) {
+      if (calc) {
+        const int yy = pix != kNoPix ? pix >> 16 : 0, xx = pix != kNoPix ? pix & 0xffff : 0;
+        const double u = col_u(xx, yy, cAu[k], cBu[k], cCu[k]);
+        int loc, hic;
+        w0 = col_window(P, u, NB, loc, hic);
+        a = w0 >= 0 ? col_value(P, cH[k], u, w0 + ub, loc, hic) : 0.0f;
+      } else {
+        const size_t r = rec_at(pix, k);
+        a = colt[r + ub];
+        w0 = __float_as_int(colt[r + NB]);
+      }
+    };
     const int *lst = list + n0;
     auto pix_at = [&](int raw, int k, int n) -> int {
       return n < cnt ? __builtin_amdgcn_readlane(raw, k) : kNoPix;
@@ -1142,11 +1170,8 @@ __device__ __forceinline__ void k_icd_mwg_body(const KP &P, const int2 cl, int i
       int w00[VPL], w01[VPL];
 #pragma unroll
       for (int k = 0; k < VPL; k++) {
-        const size_t r0 = rec_at(p0, k), r1 = rec_at(p1, k);
-        a0[k] = colt[r0 + ub];
-        w00[k] = __float_as_int(colt[r0 + NB]);
-        a1[k] = colt[r1 + ub];
-        w01[k] = __float_as_int(colt[r1 + NB]);
+        col_at(p0, k, a0[k], w00[k]);
+        col_at(p1, k, a1[k], w01[k]);
       }
       wait_vm_all();
 #pragma unroll
@@ -1178,9 +1203,7 @@ __device__ __forceinline__ void k_icd_mwg_body(const KP &P, const int2 cl, int i
         int w2[VPL];
 #pragma unroll
         for (int k = 0; k < VPL; k++) {
-          const size_t r = rec_at(p2, k);
-          a2[k] = colt[r + ub];
-          w2[k] = __float_as_int(colt[r + NB]);
+          col_at(p2, k, a2[k], w2[k]);
           nea[k] = ebase[k] + (size_t)max(nw0[k], 0) * MZ;
           ld_rows<MZ>(P.e + nea[k], ne[k]);
           ld_rows<MZ>(P.w + nea[k], nwv[k]);
