@@ -105,6 +105,10 @@ typedef struct {
                                      it (kernel 0 takes the other
                                      families), else at most this many
                                      (2 to 64) */
+  int32_t split_spin_limit;       /* polls of one cross-workgroup wait of
+                                     the split-cell kernel before the wait
+                                     is abandoned and the call fails:
+                                     0 = 2^22 (about a second, default) */
 } tb_options;
 
 void tb_get_options(tb_options *opt);
@@ -127,6 +131,14 @@ int tb_stage_create(tb_stage **out, const float *projections,
                     int32_t mult_xy, int32_t mult_z, int32_t stage_idx,
                     int32_t prev_mult_z, double var_est);
 void tb_stage_destroy(tb_stage *st);
+/* Ends the cross-workgroup waits of the split-cell kernel launch running on
+ * the stage (from any host thread, through a stream of its own): its
+ * workgroups stop waiting, finish with no further steps, and the call that
+ * launched it fails ("k_icd_split: a cross-workgroup wait timed out").  A
+ * split-cell launch that starts later clears the request; other kernels
+ * are not affected.  No counterpart in the reference (its threads' waits
+ * are OpenMP barriers). */
+int tb_stage_abort(tb_stage *st);
 
 /* ints: N_p N_r N_t N_x N_y N_z N_time mult_xy mult_z num_z_blocks
  * updateProjOffset updateVar initICD U ; dbls as the oracle's

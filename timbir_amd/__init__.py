@@ -25,7 +25,7 @@ EXPORTS = [
     "tb_cost", "tb_offset_update",
     "tb_variance_update", "tb_voxel_update", "tb_update_colour", "tb_vsc_select",
     "tb_nhicd_list", "tb_icd_iteration", "tb_stage_run", "tb_get_stats", "tb_reset_stats",
-    "tb_synchronize", "tb_stage_object_device",
+    "tb_synchronize", "tb_stage_object_device", "tb_stage_abort",
     # many problems in lockstep
     "tb_batch_create", "tb_batch_destroy", "tb_batch_iteration", "tb_batch_get_stats",
     # autotuner
@@ -46,7 +46,7 @@ class TbOptions(C.Structure):
                 ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
                 ("init_fbp", C.c_int32), ("no_column_table", C.c_int32),
                 ("theta_fp", C.c_int32), ("row_lookahead", C.c_int32), ("autotune", C.c_int32),
-                ("split_segments", C.c_int32)]
+                ("split_segments", C.c_int32), ("split_spin_limit", C.c_int32)]
 
 
 class TbStats(C.Structure):
@@ -126,6 +126,7 @@ def lib() -> C.CDLL:
     L.tb_get_stats.argtypes = [vp, C.POINTER(TbStats)]
     L.tb_reset_stats.argtypes = [vp]
     L.tb_synchronize.argtypes = [vp]
+    L.tb_stage_abort.argtypes = [vp]
     L.tb_stage_object_device.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_uint64)]
     L.tb_batch_create.argtypes = [C.POINTER(vp), i32]
     L.tb_batch_destroy.argtypes = [vp]
@@ -326,6 +327,11 @@ class Stage:
 
     def synchronize(self):
         _check(self.L.tb_synchronize(self.h))
+
+    def abort(self):
+        """End the cross-workgroup waits of the split-cell launch running
+        on this stage (tb_stage_abort); safe from another thread."""
+        _check(self.L.tb_stage_abort(self.h))
 
     def object_device(self):
         """(device address, bytes) of the object in HBM."""

@@ -83,7 +83,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -138,6 +138,7 @@ struct tb_stage {
   // the device's compute units
   unsigned long long *split_rec = nullptr;  // [slots][2][64][4] tagged partials
   unsigned *split_w = nullptr;              // err, abort
+  hipStream_t abort_stream = nullptr;       // tb_stage_abort's writes
   int split_slots = 0, n_cus = 0;
   bool split_ran = false;
   int colt_nb = 0;
@@ -260,6 +261,7 @@ static void detector_response(const float *angles, int N_p, double delta_xy, dou
 extern "C" void tb_stage_destroy(tb_stage *st) {
   if (!st) return;
   if (st->stream) (void)hipStreamSynchronize(st->stream);
+  if (st->abort_stream) (void)hipStreamDestroy(st->abort_stream);
   void *bufs[] = {st->e, st->w, st->y, st->d, st->obj, st->mag, st->hr, st->filt, st->cosv,
                   st->sinv, st->sfilt_d, st->view_off_d, st->view_idx_d, st->view_frame_d,
                   st->mask, st->nh_list, st->nh_count, st->nh_ord, st->nh_ord_count, st->ord,
@@ -737,13 +739,21 @@ static hipEvent_t *ev_pair(tb_stage *st) {
 
 static int harvest_events(tb_stage *st) {
   if (st->split_ran) {
-    // a cross-workgroup wait of the split-cell kernel that timed out
+    // a cross-workgroup wait of any split-cell launch since the last check
+    // that timed out (the word only accumulates on the device; it is
+    // cleared here, after it has been read)
     unsigned err = 0;
     HIPCHK(hipMemcpyAsync(&err, st->split_w, sizeof(err),
                           hipMemcpyDeviceToHost, st->stream));
     HIPCHK(hipStreamSynchronize(st->stream));
     st->split_ran = false;
-    if (err) return fail("k_icd_split: a cross-workgroup wait timed out (results invalid)");
+    if (err) {
+      hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, st->stream,
+                         reinterpret_cast<float *>(st->split_w), (size_t)1, 0.0f);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st->stream));
+      return fail("k_icd_split: a cross-workgroup wait timed out (results invalid)");
+    }
   }
   if (st->ev_used == 0) return 0;
   HIPCHK(hipEventSynchronize(st->ev[st->ev_used - 1]));
@@ -1082,7 +1092,8 @@ static int launch_split(tb_stage *st, const int2 *cells, int ncells, int iter, i
   B.abort = st->split_w + 1;
   B.S = S;
   B.smax = kSplitSegMax;
-  B.spin_limit = 1u << 22;  // about a second per wait
+  // polls per wait: 2^22 (about a second) unless the option asks fewer
+  B.spin_limit = g_opt.split_spin_limit > 0 ? (unsigned)g_opt.split_spin_limit - 1u : 1u << 22;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDevices) return fail("launch_split: device %d", dev);
@@ -2243,5 +2254,16 @@ extern "C" int tb_stage_object_device(tb_stage *st, void **ptr, uint64_t *bytes)
 
 extern "C" int tb_synchronize(tb_stage *st) {
   HIPCHK(hipStreamSynchronize(st->stream));
+  return 0;
+}
+
+extern "C" int tb_stage_abort(tb_stage *st) {
+  if (!st) return fail("tb_stage_abort: no stage");
+  std::lock_guard<std::mutex> guard(g_split_mu);
+  if (!st->split_w) return 0;  // no split-cell launch yet
+  if (!st->abort_stream) HIPCHK(hipStreamCreateWithFlags(&st->abort_stream, hipStreamNonBlocking));
+  static const unsigned one = 1u;
+  HIPCHK(hipMemcpyAsync(st->split_w + 1, &one, sizeof(one), hipMemcpyHostToDevice, st->abort_stream));
+  HIPCHK(hipStreamSynchronize(st->abort_stream));
   return 0;
 }

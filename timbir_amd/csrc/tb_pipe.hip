@@ -2486,9 +2486,11 @@ template __global__ void k_icd_wide_m<17, 2, 8>(const KP *, const int4 *, int, i
 // neighbours).  Zero-skipped voxels are decided from those values alike and
 // exchange nothing.
 // Every wait is bounded (spin_limit polls) and also ends when the host's
-// abort word is nonzero: then the workgroup records the timeout in err, takes
+// abort word is nonzero (tb_stage_abort; the reset before each launch
+// clears it): then the workgroup records the timeout in err, takes
 // dx = 0 for the rest of the sequence and stops waiting, so every wave still
-// reaches the end.  Records are double-buffered by voxel parity (a workgroup
+// reaches the end; the error word collects the timeouts of every launch
+// until the host reads and clears it.  Records are double-buffered by voxel parity (a workgroup
 // publishes voxel n + 2's record only after every workgroup has published
 // voxel n + 1's, which each does after it has read voxel n's) and zeroed
 // before each launch.
@@ -2502,7 +2504,9 @@ __global__ void k_split_reset(SplitBuf B, int slots) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     B.rec[i] = 0ull;
-  if (blockIdx.x == 0 && threadIdx.x == 0) B.err[0] = 0u;
+  // the error word is left alone: it collects every launch's timeouts until
+  // the host reads it
+  if (blockIdx.x == 0 && threadIdx.x == 0) *const_cast<unsigned *>(B.abort) = 0u;
 }
 
 template <int NBX, int R, int MAXTH>
