@@ -41,6 +41,8 @@ WORKLOAD = ("4D interlaced-view MBIR, deployment scan, finest stage: 512x512x32 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_k_icd_mwg_deploy4d_512x512x32x16.md")
 GOLDEN = os.path.join(ROOT, "tests", "golden", "converged_deploy4d.npz")
 WEIGHTS = os.path.join(ROOT, "timbir_amd", "weights", "unet_init.npz")
+# the network trained on deployment-size scans of the same model (other seeds)
+WEIGHTS_DEPLOY = os.path.join(ROOT, "timbir_amd", "weights", "unet_init_deploy4d.npz")
 
 
 def measured_traffic():
@@ -247,9 +249,11 @@ def main():
         net = None
         starts = [x for x in args.starts.split(",") if x]
         if "learned" in starts:
-            if os.path.exists(WEIGHTS) and not args.small and args.workload != "precision3d":
+            w = (WEIGHTS_DEPLOY if args.workload == "deploy4d" and os.path.exists(WEIGHTS_DEPLOY)
+                 else WEIGHTS)
+            if os.path.exists(w) and not args.small and args.workload != "precision3d":
                 from timbir_amd.learned import UNet
-                net = UNet(params=UNet.load_params(WEIGHTS))
+                net = UNet(params=UNet.load_params(w))
             else:
                 starts.remove("learned")
         until = ("rel_change",) if args.until == "rel_change" else ("rel_change", "map")

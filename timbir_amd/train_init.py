@@ -19,6 +19,14 @@ over frames.  One scan gives 16 samples (its frames).
 
     python -m timbir_amd.train_init --scans 64 --steps 3000 \\
         --out timbir_amd/weights/unet_init.npz
+
+With --workload deploy4d the training scans are the deployment scan's model
+at its own size (datagen.deploy4d: 512 x 512 x 32 slices x 16 frames, 2048
+views, I0 = 1e4) with other seeds; the scan the bench and the tests use
+(seed 20260005) is never one of them.
+
+    python -m timbir_amd.train_init --workload deploy4d --scans 6 --val 1 \\
+        --steps 3000 --out timbir_amd/weights/unet_init_deploy4d.npz
 """
 from __future__ import annotations
 
@@ -132,15 +140,17 @@ def fbp_and_target(scan: dict, finest_iters: int, coarse_rel: float = 1e-5,
 
 def crops(data, rng, batch: int, crop: int):
     """A batch of random crops (every slice, crop x crop pixels) of random
-    frames: (fbp [B][NZ][c][c], mean, target)."""
+    frames: (fbp [B][NZ][c][c], mean, target).  data holds (fbp, its mean
+    over frames, target) per scan."""
     fb, mn, tg = [], [], []
     for _ in range(batch):
-        fbp, target = data[int(rng.integers(len(data)))]
-        f = int(rng.integers(FRAMES))
-        y0, x0 = rng.integers(0, N - crop + 1, 2)
+        fbp, mean, target = data[int(rng.integers(len(data)))]
+        frames, n = fbp.shape[0], fbp.shape[-1]
+        f = int(rng.integers(frames))
+        y0, x0 = rng.integers(0, n - crop + 1, 2)
         sl = (slice(None), slice(y0, y0 + crop), slice(x0, x0 + crop))
         fb.append(fbp[f][sl])
-        mn.append(fbp.mean(axis=0)[sl].astype(np.float32))
+        mn.append(mean[sl])
         tg.append(target[f][sl])
     return np.stack(fb), np.stack(mn), np.stack(tg)
 
@@ -174,9 +184,9 @@ def evaluate(net: UNet, val) -> dict:
     """Mean squared error to the target of the back-projection and of the
     network's output over the validation scans' full frames."""
     e_fbp, e_net = [], []
-    for fbp, target in val:
+    for fbp, mean, target in val:
         f = DevArray.from_numpy(np.ascontiguousarray(fbp))
-        m = DevArray.from_numpy(np.ascontiguousarray(fbp.mean(axis=0), np.float32))
+        m = DevArray.from_numpy(np.ascontiguousarray(mean))
         out = net.forward(f, m).numpy()
         e_fbp.append(float(np.mean((fbp - target) ** 2)))
         e_net.append(float(np.mean((out - target) ** 2)))
@@ -185,6 +195,7 @@ def evaluate(net: UNet, val) -> dict:
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--workload", default="interlaced256", choices=["interlaced256", "deploy4d"])
     ap.add_argument("--scans", type=int, default=64)
     ap.add_argument("--val", type=int, default=2)
     ap.add_argument("--finest-iters", type=int, default=20)
@@ -209,10 +220,14 @@ def main(argv=None):
             log(f"data budget reached after {len(data)} scans")
             break
         ts = time.time()
-        scan = synth_scan(a.seed + k)
+        if a.workload == "deploy4d":
+            scan = datagen.deploy4d(seed=a.seed + k)
+        else:
+            scan = synth_scan(a.seed + k)
         tg = time.time() - ts
         fbp, target = fbp_and_target(scan, a.finest_iters)
-        data.append((fbp, target))
+        del scan
+        data.append((fbp, fbp.mean(axis=0).astype(np.float32), target))
         log(f"scan {k}: generated in {tg:.1f} s, reconstructed in {time.time() - ts - tg:.1f} s "
             f"(mse fbp {np.mean((fbp - target) ** 2):.3e})")
     val, train_set = data[:a.val], data[a.val:]
