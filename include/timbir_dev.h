@@ -134,9 +134,10 @@ void tb_stage_destroy(tb_stage *st);
 /* Ends the cross-workgroup waits of the split-cell kernel launch running on
  * the stage (from any host thread, through a stream of its own): its
  * workgroups stop waiting, finish with no further steps, and the call that
- * launched it fails ("k_icd_split: a cross-workgroup wait timed out").  A
- * split-cell launch that starts later clears the request; other kernels
- * are not affected.  No counterpart in the reference (its threads' waits
+ * launched it fails ("k_icd_split: a cross-workgroup wait timed out"); the
+ * stage's object and error sinogram are then invalid and the stage is to be
+ * destroyed.  A split-cell launch that starts later clears the request;
+ * other stages are not affected.  No counterpart in the reference (its threads' waits
  * are OpenMP barriers). */
 int tb_stage_abort(tb_stage *st);
 
