@@ -109,6 +109,15 @@ typedef struct {
                                      the split-cell kernel before the wait
                                      is abandoned and the call fails:
                                      0 = 2^22 (about a second, default) */
+  int32_t hicd_order;             /* the order a HICD chunk's pixels are
+                                     visited in: each chunk holds the pixels
+                                     of one range of the cell's keyed random
+                                     permutation (as the reference's chunks
+                                     hold ranges of its random list,
+                                     XT_ICD_update.c:983-986); 0 = in raster
+                                     order (default: neighbouring voxels
+                                     share cache lines), 1 = in permutation
+                                     order */
 } tb_options;
 
 void tb_get_options(tb_options *opt);

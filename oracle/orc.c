@@ -105,6 +105,26 @@ uint32_t orc_perm(uint32_t n, uint32_t N, uint32_t key) {
   return n;
 }
 
+/* inverse of orc_perm: the rounds undone in reverse order, cycle-walking
+ * alike (a round maps (L, R) to (R, L ^ F(R))) */
+uint32_t orc_perm_inv(uint32_t n, uint32_t N, uint32_t key) {
+  uint32_t bits = 1;
+  while ((1u << bits) < N) bits++;
+  uint32_t h = (bits + 1) / 2;
+  uint32_t mask = (1u << h) - 1u;
+  do {
+    uint32_t L = n >> h, R = n & mask;
+    for (int r = 3; r >= 0; r--) {
+      uint32_t F = mix32(L ^ (key + (uint32_t)r * 0x9e3779b9U)) & mask;
+      uint32_t pl = R ^ F;
+      R = L;
+      L = pl;
+    }
+    n = (L << h) | R;
+  } while (n >= N);
+  return n;
+}
+
 /* ======================================================================= */
 /* System model                                                            */
 /* ======================================================================= */
@@ -1015,12 +1035,24 @@ int32_t orc_nhicd_count(const orc_problem *p, int32_t f, int32_t blk, int32_t *x
 }
 
 /* colour schedule: the lines of cell (f, z) for one chunk, as the GPU walks
- * them: permutation indices [n0, n1), unmasked pixels skipped. */
+ * them: the unmasked pixels of permutation indices [n0, n1), in raster order
+ * (hicd_order 0) or in permutation order (1). */
 static int32_t colour_chunk(const orc_problem *p, int32_t iter, int32_t f, int32_t z,
                             int32_t n0, int32_t n1, int32_t *xs, int32_t *ys) {
   uint32_t N = (uint32_t)(p->N_x * p->N_y);
   uint32_t key = orc_perm_key(p->opt.order_seed, iter, f, z);
   int32_t cnt = 0;
+  if (p->opt.hicd_order == 0) {
+    for (uint32_t pix = 0; pix < N; pix++) {
+      if (p->mask[pix] != 1) continue;
+      int32_t n = (int32_t)orc_perm_inv(pix, N, key);
+      if (n < n0 || n >= n1) continue;
+      xs[cnt] = (int32_t)(pix % (uint32_t)p->N_x);
+      ys[cnt] = (int32_t)(pix / (uint32_t)p->N_x);
+      cnt++;
+    }
+    return cnt;
+  }
   for (int32_t n = n0; n < n1; n++) {
     uint32_t pix = orc_perm((uint32_t)n, N, key);
     int32_t yy = (int32_t)(pix / (uint32_t)p->N_x), xx = (int32_t)(pix % (uint32_t)p->N_x);
@@ -1030,6 +1062,11 @@ static int32_t colour_chunk(const orc_problem *p, int32_t iter, int32_t f, int32
     cnt++;
   }
   return cnt;
+}
+
+int32_t orc_colour_chunk(const orc_problem *p, int32_t iter, int32_t f, int32_t z, int32_t n0,
+                         int32_t n1, int32_t *xs, int32_t *ys) {
+  return colour_chunk(p, iter, f, z, n0, n1, xs, ys);
 }
 
 /* ======================================================================= */

@@ -42,6 +42,8 @@ typedef struct {
   int32_t write_files;      /* write the reference's .bin outputs into cwd */
   int32_t no_nhicd;         /* NO_NHICD (XT_Constants.h:91) */
   uint32_t order_seed;      /* colour schedule permutation seed */
+  int32_t hicd_order;       /* colour schedule: a HICD chunk's pixels in raster
+                               order (0) or in permutation order (1) */
 } orc_options;
 
 /* ---- RNG: randlib.c:119-174 -------------------------------------------- */
@@ -51,6 +53,7 @@ void orc_random2_stream(uint32_t seed, int64_t n, double *out);
 /* ---- permutation used by the colour schedule ----------------------------- */
 uint32_t orc_perm_key(uint32_t seed, int32_t iter, int32_t frame, int32_t slice);
 uint32_t orc_perm(uint32_t n, uint32_t N, uint32_t key);
+uint32_t orc_perm_inv(uint32_t n, uint32_t N, uint32_t key);
 
 /* ---- system model ------------------------------------------------------ */
 /* XT_Profile.c:47-176 (DetectorResponseProfile) -> H_r[N_p][65] */
@@ -92,6 +95,12 @@ orc_problem *orc_problem_new(const float *projections, const float *weights,
                              int32_t stage_idx, int32_t prev_mult_z,
                              double var_est, const orc_options *opt);
 void orc_problem_free(orc_problem *p);
+
+/* The pixels of cell (f, z)'s HICD chunk of permutation indices [n0, n1) in
+ * the colour schedule's visit order (p's hicd_order); returns the count.
+ * xs and ys hold N_x * N_y entries. */
+int32_t orc_colour_chunk(const orc_problem *p, int32_t iter, int32_t f, int32_t z, int32_t n0,
+                         int32_t n1, int32_t *xs, int32_t *ys);
 
 /* Scalars of the stage, for tests.  ints: N_p N_r N_t N_x N_y N_z N_time
  * mult_xy mult_z num_z_blocks updateProjOffset updateVar initICD U(masked)

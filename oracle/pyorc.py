@@ -33,6 +33,7 @@ class Options(C.Structure):
         ("write_files", C.c_int32),
         ("no_nhicd", C.c_int32),
         ("order_seed", C.c_uint32),
+        ("hicd_order", C.c_int32),
     ]
 
 
@@ -63,6 +64,11 @@ def lib(precision: str = "f32") -> C.CDLL:
     L.orc_perm_key.argtypes = [C.c_uint32, C.c_int32, C.c_int32, C.c_int32]
     L.orc_perm.restype = C.c_uint32
     L.orc_perm.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_perm_inv.restype = C.c_uint32
+    L.orc_perm_inv.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    L.orc_colour_chunk.restype = C.c_int32
+    L.orc_colour_chunk.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                   _i32p, _i32p]
     L.orc_detector_response.argtypes = [_f32p, C.c_int32, C.c_double, C.c_double,
                                         C.c_double, C.c_double, _f32p]
     L.orc_amatrix_column.restype = C.c_int32
@@ -147,9 +153,9 @@ INFO_DBLS = ["delta_r", "delta_t", "R0", "RMax", "T0", "x0", "y0", "delta_xy", "
 
 
 def options(schedule=SCHED_COLOUR, num_threads=8, compute_cost=0, abort=0, max_iterations=0,
-            write_files=0, no_nhicd=0, order_seed=0) -> Options:
+            write_files=0, no_nhicd=0, order_seed=0, hicd_order=0) -> Options:
     return Options(schedule, num_threads, compute_cost, abort, max_iterations, write_files,
-                   no_nhicd, order_seed)
+                   no_nhicd, order_seed, hicd_order)
 
 
 class Problem:

@@ -247,6 +247,42 @@ def test_reference_and_colour_schedules_share_the_fixed_point():
     assert np.max(np.abs(out[0][1] - out[1][1])) < 1e-7
 
 
+def test_permutation_inverse_and_chunk_orders():
+    """The colour schedule's keyed permutation is a bijection whose inverse
+    undoes it, for powers of two and cycle-walked sizes; a HICD chunk in
+    raster order (hicd_order 0) holds exactly the unmasked pixels of its
+    permutation-index range (hicd_order 1's chunk), sorted by pixel index,
+    and the K chunks partition the unmasked pixels."""
+    lib = L()
+    for N in (1000, 4096, 65536, 5000):
+        key = lib.orc_perm_key(20260001, 3, 2, 5)
+        fwd = np.array([lib.orc_perm(n, N, key) for n in range(N)], np.int64)
+        assert np.array_equal(np.sort(fwd), np.arange(N))
+        inv = np.array([lib.orc_perm_inv(int(p), N, key) for p in fwd], np.int64)
+        assert np.array_equal(inv, np.arange(N))
+    scan = small_scan(N=24, n_views=16)
+    probs = {o: pyorc.Problem(scan, opt=pyorc.options(hicd_order=o)) for o in (0, 1)}
+    p0 = probs[0]
+    nx = p0.info["N_x"]
+    nxy = nx * p0.info["N_y"]
+    mask = p0.mask().reshape(-1)
+    K = 10
+    for it, f, z in ((1, 0, 0), (2, 1, 3)):
+        seen = []
+        for j in range(K):
+            n0, n1 = j * (nxy // K), nxy if j == K - 1 else (j + 1) * (nxy // K)
+            got = {}
+            for o, q in probs.items():
+                xs = np.zeros(nxy, np.int32)
+                ys = np.zeros(nxy, np.int32)
+                n = lib.orc_colour_chunk(q.h, it, f, z, n0, n1, xs, ys)
+                got[o] = ys[:n].astype(np.int64) * nx + xs[:n]
+            assert np.array_equal(got[0], np.sort(got[1]))
+            seen.append(got[0])
+        allp = np.concatenate(seen)
+        assert np.array_equal(np.sort(allp), np.flatnonzero(mask == 1))
+
+
 def _state(precision="f64", **kw):
     scan = small_scan(N=16, n_views=24, rings=0.02, **kw)
     p = pyorc.Problem(scan, precision=precision)

@@ -46,7 +46,8 @@ class TbOptions(C.Structure):
                 ("kernel", C.c_int32), ("views_per_lane", C.c_int32),
                 ("init_fbp", C.c_int32), ("no_column_table", C.c_int32),
                 ("theta_fp", C.c_int32), ("row_lookahead", C.c_int32), ("autotune", C.c_int32),
-                ("split_segments", C.c_int32), ("split_spin_limit", C.c_int32)]
+                ("split_segments", C.c_int32), ("split_spin_limit", C.c_int32),
+                ("hicd_order", C.c_int32)]
 
 
 class TbStats(C.Structure):

@@ -68,6 +68,7 @@ struct KP {
   double *cell_voxmag;
   unsigned long long *cell_upd, *cell_zero, *cell_ent;
   uint32_t order_seed;
+  int hicd_order;  // 0: a chunk's pixels in raster order, 1: in permutation order
 };
 
 // LDS bytes of the LDS-resident voxel-update kernel for a cell of nv views:
@@ -124,6 +125,26 @@ TB_HD uint32_t perm(uint32_t n, uint32_t N, uint32_t key) {
       uint32_t nl = R;
       R = L ^ F;
       L = nl;
+    }
+    n = (L << h) | R;
+  } while (n >= N);
+  return n;
+}
+
+// The inverse of perm: the Feistel rounds backwards, cycle-walking alike.
+TB_HD uint32_t perm_inv(uint32_t n, uint32_t N, uint32_t key) {
+  uint32_t bits = 1;
+  while ((1u << bits) < N) bits++;
+  uint32_t h = (bits + 1) / 2;
+  uint32_t mask = (1u << h) - 1u;
+  do {
+    uint32_t L = n >> h, R = n & mask;
+    for (int r = 3; r >= 0; r--) {
+      // a round maps (L, R) to (R, L ^ F(R)); undo it from (L, R) = (R0, L0 ^ F(R0))
+      uint32_t F = mix32(L ^ (key + (uint32_t)r * 0x9e3779b9U)) & mask;
+      uint32_t pl = R ^ F;
+      R = L;
+      L = pl;
     }
     n = (L << h) | R;
   } while (n >= N);

@@ -83,7 +83,7 @@ using namespace tb;
 // errors and options
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
-static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+static tb_options g_opt = {0, 0, kMaxIters, 0, 1, 0u, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 static int fail(const char *fmt, ...) {
   char buf[512];
@@ -187,7 +187,7 @@ struct tb_stage {
       P.T2l = (float)(t2 - (double)P.T2h);
     }
     P.cell_voxmag = cell_voxmag; P.cell_upd = cell_upd; P.cell_zero = cell_zero;
-    P.cell_ent = cell_ent; P.order_seed = g_opt.order_seed;
+    P.cell_ent = cell_ent; P.order_seed = g_opt.order_seed; P.hicd_order = g_opt.hicd_order;
     return P;
   }
   size_t nsino() const { return (size_t)N_p * N_r * N_t; }

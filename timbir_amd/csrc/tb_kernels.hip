@@ -360,11 +360,22 @@ __global__ __launch_bounds__(NT) void k_update_voxels(KP P, const int2 *__restri
     }
   }
 
-  for (int n = n0; n < n1; n++) {
+  // a HICD chunk in raster order (hicd_order 0): every pixel in turn, those
+  // outside the chunk's permutation-index range skipped (uniformly)
+  const bool raster = seq == 0 && P.hicd_order == 0;
+  for (int n = raster ? 0 : n0; n < (raster ? Nxy : n1); n++) {
     int pix;
-    if (seq == 0) pix = (int)perm((uint32_t)n, (uint32_t)Nxy, key);
-    else if (seq == 1) pix = P.nh_list[(size_t)cell * P.nh_cap + n];
-    else pix = pix1;
+    if (raster) {
+      pix = n;
+      const int q = (int)perm_inv((uint32_t)pix, (uint32_t)Nxy, key);
+      if (q < n0 || q >= n1) continue;
+    } else if (seq == 0) {
+      pix = (int)perm((uint32_t)n, (uint32_t)Nxy, key);
+    } else if (seq == 1) {
+      pix = P.nh_list[(size_t)cell * P.nh_cap + n];
+    } else {
+      pix = pix1;
+    }
     const int yy = pix / P.N_x, xx = pix - yy * P.N_x;
     if (P.mask[pix] != 1) {
       if (seq == 1 && tid == 0) magc[pix] = 0.0f;  // XT_VoxUpdate.c:156

@@ -35,11 +35,14 @@
 
 namespace tb {
 
-// Per-iteration voxel order of every cell: the keyed permutation of the
-// N_x*N_y pixels with unmasked pixels dropped, as (y << 16) | x, plus where
-// each of the K HICD chunks starts (chunks split the permutation index range,
-// as XT_ICD_update.c:983-986 splits the reference's list).  One workgroup per
-// cell.
+// Per-iteration voxel order of every cell, and where each of the K HICD
+// chunks starts.  Chunk j holds the unmasked pixels whose index in the
+// cell's keyed permutation of the N_x*N_y pixels lies in [j per, (j + 1)
+// per) (the last chunk to the end), as XT_ICD_update.c:983-986 splits the
+// reference's random list; pixels are (y << 16) | x.  With hicd_order 0 a
+// chunk's pixels are listed in raster order (consecutive voxels then share
+// the cache lines of their sinogram rows, column records and neighbours),
+// with 1 in permutation order.  One workgroup per cell.
 __global__ __launch_bounds__(256) void k_build_order(KP P, int iter) {
   const int cell = blockIdx.x;
   const int f = cell / P.N_z, z = cell - f * P.N_z;
@@ -51,14 +54,8 @@ __global__ __launch_bounds__(256) void k_build_order(KP P, int iter) {
   __shared__ int s_wcnt[4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int base = 0;
-  for (int start = 0; start < Nxy; start += blockDim.x) {
-    const int n = start + threadIdx.x;
-    int pix = 0;
-    bool ok = false;
-    if (n < Nxy) {
-      pix = (int)perm((uint32_t)n, (uint32_t)Nxy, key);
-      ok = P.mask[pix] == 1;
-    }
+  // appends this thread's pixel (when ok) after `base`, in thread order
+  auto append = [&](bool ok, int pix) -> int {
     const unsigned long long bal = __ballot(ok);
     const int inwave = __popcll(bal & ((1ull << lane) - 1ull));
     if (lane == 0) s_wcnt[wid] = __popcll(bal);
@@ -67,11 +64,39 @@ __global__ __launch_bounds__(256) void k_build_order(KP P, int iter) {
     for (int w = 0; w < wid; w++) pos += s_wcnt[w];
     pos += inwave;
     if (ok && pos < P.ord_cap) ord[pos] = ((pix / P.N_x) << 16) | (pix % P.N_x);
-    // chunk j starts at permutation index j * per; the count of masked
-    // pixels before index n is pos (before this thread's own pixel)
-    if (n < Nxy && per > 0 && n % per == 0 && n / per < P.K) off[n / per] = pos;
-    for (int w = 0; w < 4; w++) base += s_wcnt[w];
-    __syncthreads();
+    return pos;
+  };
+  if (P.hicd_order == 0) {
+    for (int j = 0; j < P.K; j++) {
+      if (threadIdx.x == 0) off[j] = min(base, P.ord_cap);
+      for (int start = 0; start < Nxy; start += blockDim.x) {
+        const int pix = start + threadIdx.x;
+        bool ok = false;
+        if (pix < Nxy && P.mask[pix] == 1) {
+          const int n = (int)perm_inv((uint32_t)pix, (uint32_t)Nxy, key);
+          ok = (per > 0 ? min(n / per, P.K - 1) : 0) == j;
+        }
+        append(ok, pix);
+        for (int w = 0; w < 4; w++) base += s_wcnt[w];
+        __syncthreads();
+      }
+    }
+  } else {
+    for (int start = 0; start < Nxy; start += blockDim.x) {
+      const int n = start + threadIdx.x;
+      int pix = 0;
+      bool ok = false;
+      if (n < Nxy) {
+        pix = (int)perm((uint32_t)n, (uint32_t)Nxy, key);
+        ok = P.mask[pix] == 1;
+      }
+      const int pos = append(ok, pix);
+      // chunk j starts at permutation index j * per; the count of masked
+      // pixels before index n is pos (before this thread's own pixel)
+      if (n < Nxy && per > 0 && n % per == 0 && n / per < P.K) off[n / per] = pos;
+      for (int w = 0; w < 4; w++) base += s_wcnt[w];
+      __syncthreads();
+    }
   }
   if (threadIdx.x == 0) off[P.K] = min(base, P.ord_cap);
 }
