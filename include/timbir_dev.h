@@ -114,10 +114,10 @@ typedef struct {
                                      of one range of the cell's keyed random
                                      permutation (as the reference's chunks
                                      hold ranges of its random list,
-                                     XT_ICD_update.c:983-986); 0 = in raster
-                                     order (default: neighbouring voxels
-                                     share cache lines), 1 = in permutation
-                                     order */
+                                     XT_ICD_update.c:983-986); 0 = in
+                                     permutation order (default), 1 = in
+                                     raster order (neighbouring voxels share
+                                     cache lines: faster on large cells) */
 } tb_options;
 
 void tb_get_options(tb_options *opt);

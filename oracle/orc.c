@@ -1035,14 +1035,14 @@ int32_t orc_nhicd_count(const orc_problem *p, int32_t f, int32_t blk, int32_t *x
 }
 
 /* colour schedule: the lines of cell (f, z) for one chunk, as the GPU walks
- * them: the unmasked pixels of permutation indices [n0, n1), in raster order
- * (hicd_order 0) or in permutation order (1). */
+ * them: the unmasked pixels of permutation indices [n0, n1), in permutation
+ * order (hicd_order 0) or in raster order (1). */
 static int32_t colour_chunk(const orc_problem *p, int32_t iter, int32_t f, int32_t z,
                             int32_t n0, int32_t n1, int32_t *xs, int32_t *ys) {
   uint32_t N = (uint32_t)(p->N_x * p->N_y);
   uint32_t key = orc_perm_key(p->opt.order_seed, iter, f, z);
   int32_t cnt = 0;
-  if (p->opt.hicd_order == 0) {
+  if (p->opt.hicd_order == 1) {
     for (uint32_t pix = 0; pix < N; pix++) {
       if (p->mask[pix] != 1) continue;
       int32_t n = (int32_t)orc_perm_inv(pix, N, key);

@@ -42,8 +42,8 @@ typedef struct {
   int32_t write_files;      /* write the reference's .bin outputs into cwd */
   int32_t no_nhicd;         /* NO_NHICD (XT_Constants.h:91) */
   uint32_t order_seed;      /* colour schedule permutation seed */
-  int32_t hicd_order;       /* colour schedule: a HICD chunk's pixels in raster
-                               order (0) or in permutation order (1) */
+  int32_t hicd_order;       /* colour schedule: a HICD chunk's pixels in
+                               permutation order (0) or in raster order (1) */
 } orc_options;
 
 /* ---- RNG: randlib.c:119-174 -------------------------------------------- */

@@ -316,6 +316,36 @@ def test_icd_iterations_track_oracle(quad, rings, kernel):
     check_kernel(g, kernel)
 
 
+@pytest.mark.parametrize("k", [0, 1, 4])
+def test_raster_hicd_order_tracks_oracle(k):
+    """tb_options.hicd_order = 1 (each HICD chunk's pixels in raster order)
+    against the oracle's colour schedule with the same order: the same
+    trajectory check as test_icd_iterations_track_oracle, for the automatic,
+    one-voxel and global-e kernels."""
+    old = tb.get_options()
+    tb.set_options(hicd_order=1, kernel=k)
+    try:
+        scan = scan_small(N=32, nz=4, n_views=40, quad_convex=0, rings=0.01, zingers=1e-3,
+                          remove_rings=2, c_s=1.0, c_t=1.0)
+        o = pyorc.Problem(scan, stage_idx=1, prev_mult_z=1,
+                          opt=pyorc.options(schedule=pyorc.SCHED_COLOUR, hicd_order=1))
+        g = tb.Stage(scan, stage_idx=1, prev_mult_z=1)
+        o.init_error_sino()
+        g.init_error_sino()
+        for it in range(1, 5):
+            o.icd_iteration(it)
+            g.icd_iteration(it)
+            xo, xg = o.obj(), g.object()
+            rmse = np.sqrt(np.mean((xg - xo) ** 2)) / (xo.max() - xo.min())
+            co, cg = o.cost()[0], g.cost()[0]
+            log(f"  raster order kernel {k} it {it}: rmse/range {rmse:.2e} cost {cg:.8e} vs {co:.8e}")
+            assert rmse < 1e-4
+            assert abs(cg - co) <= 1e-5 * abs(co)
+        g.close()
+    finally:
+        tb.set_options(hicd_order=old.hicd_order, kernel=old.kernel)
+
+
 def test_converges_to_map(kernel):
     """GPU ICD reaches the scipy minimiser of the explicit MAP cost."""
     import mathref

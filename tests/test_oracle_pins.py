@@ -250,8 +250,8 @@ def test_reference_and_colour_schedules_share_the_fixed_point():
 def test_permutation_inverse_and_chunk_orders():
     """The colour schedule's keyed permutation is a bijection whose inverse
     undoes it, for powers of two and cycle-walked sizes; a HICD chunk in
-    raster order (hicd_order 0) holds exactly the unmasked pixels of its
-    permutation-index range (hicd_order 1's chunk), sorted by pixel index,
+    raster order (hicd_order 1) holds exactly the unmasked pixels of its
+    permutation-index range (hicd_order 0's chunk), sorted by pixel index,
     and the K chunks partition the unmasked pixels."""
     lib = L()
     for N in (1000, 4096, 65536, 5000):
@@ -277,8 +277,8 @@ def test_permutation_inverse_and_chunk_orders():
                 ys = np.zeros(nxy, np.int32)
                 n = lib.orc_colour_chunk(q.h, it, f, z, n0, n1, xs, ys)
                 got[o] = ys[:n].astype(np.int64) * nx + xs[:n]
-            assert np.array_equal(got[0], np.sort(got[1]))
-            seen.append(got[0])
+            assert np.array_equal(got[1], np.sort(got[0]))
+            seen.append(got[1])
         allp = np.concatenate(seen)
         assert np.array_equal(np.sort(allp), np.flatnonzero(mask == 1))
 

@@ -68,7 +68,7 @@ struct KP {
   double *cell_voxmag;
   unsigned long long *cell_upd, *cell_zero, *cell_ent;
   uint32_t order_seed;
-  int hicd_order;  // 0: a chunk's pixels in raster order, 1: in permutation order
+  int hicd_order;  // 0: a chunk's pixels in permutation order, 1: in raster order
 };
 
 // LDS bytes of the LDS-resident voxel-update kernel for a cell of nv views:

@@ -360,9 +360,9 @@ __global__ __launch_bounds__(NT) void k_update_voxels(KP P, const int2 *__restri
     }
   }
 
-  // a HICD chunk in raster order (hicd_order 0): every pixel in turn, those
+  // a HICD chunk in raster order (hicd_order 1): every pixel in turn, those
   // outside the chunk's permutation-index range skipped (uniformly)
-  const bool raster = seq == 0 && P.hicd_order == 0;
+  const bool raster = seq == 0 && P.hicd_order == 1;
   for (int n = raster ? 0 : n0; n < (raster ? Nxy : n1); n++) {
     int pix;
     if (raster) {

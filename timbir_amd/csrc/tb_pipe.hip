@@ -40,9 +40,9 @@ namespace tb {
 // cell's keyed permutation of the N_x*N_y pixels lies in [j per, (j + 1)
 // per) (the last chunk to the end), as XT_ICD_update.c:983-986 splits the
 // reference's random list; pixels are (y << 16) | x.  With hicd_order 0 a
-// chunk's pixels are listed in raster order (consecutive voxels then share
-// the cache lines of their sinogram rows, column records and neighbours),
-// with 1 in permutation order.  One workgroup per cell.
+// chunk's pixels are listed in permutation order, with 1 in raster order
+// (consecutive voxels then share the cache lines of their sinogram rows,
+// column records and neighbours).  One workgroup per cell.
 __global__ __launch_bounds__(256) void k_build_order(KP P, int iter) {
   const int cell = blockIdx.x;
   const int f = cell / P.N_z, z = cell - f * P.N_z;
@@ -66,7 +66,7 @@ __global__ __launch_bounds__(256) void k_build_order(KP P, int iter) {
     if (ok && pos < P.ord_cap) ord[pos] = ((pix / P.N_x) << 16) | (pix % P.N_x);
     return pos;
   };
-  if (P.hicd_order == 0) {
+  if (P.hicd_order == 1) {
     for (int j = 0; j < P.K; j++) {
       if (threadIdx.x == 0) off[j] = min(base, P.ord_cap);
       for (int start = 0; start < Nxy; start += blockDim.x) {
