@@ -163,6 +163,9 @@ def main():
                          "reference's stop rule at every stage")
     ap.add_argument("--kernel", type=int, default=0,
                     help="voxel-update kernel (tb_options.kernel; 0: the fastest that applies)")
+    ap.add_argument("--hicd-order", type=int, default=0, choices=[0, 1],
+                    help="HICD chunk visit order (tb_options.hicd_order): 0 permutation order "
+                         "(default), 1 raster order within each chunk's random subset")
     ap.add_argument("--row-lookahead", type=int, default=0,
                     help="voxels' (e, w) rows in flight in the deep-pipeline kernel (0: default)")
     args = ap.parse_args()
@@ -176,7 +179,8 @@ def main():
 
     import timbir_amd as tb
     from timbir_amd.batch import multires_schedule
-    tb.set_options(verbose=0, write_files=0, kernel=args.kernel, row_lookahead=args.row_lookahead)
+    tb.set_options(verbose=0, write_files=0, kernel=args.kernel, row_lookahead=args.row_lookahead,
+                   hicd_order=args.hicd_order)
     t0 = time.perf_counter()
     scan = make_scan(args.small, args.workload)
     log(f"scan generated in {time.perf_counter() - t0:.1f} s")
