@@ -2716,7 +2716,7 @@ __global__ __launch_bounds__(64 * (MAXTH + 1)) void k_icd_split(KP P, const int2
     if (lane == 0) S.nent[wave] = (unsigned long long)nsum;
   } else {
     // ---------------- exchange wave: the prior, the exchange, the solve ----------------
-    const bool lead = seg == 0;
+    const bool lead = seg == 0;  // the workgroup that keeps the cell's update counts
     const int ln = lane;
     PriorLane C;
     C.ln = ln;
