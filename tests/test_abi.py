@@ -104,3 +104,14 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "exactly one GPU" in r.stdout
+
+
+def test_bench_traffic_comes_from_the_committed_counter_summary():
+    # the roofline's HBM traffic per launch is read from the counter summary
+    # in profiles/ (FETCH_SIZE + WRITE_SIZE, copy-calibrated); it must parse
+    # and exceed the kernel's algorithmic bytes per launch (1.85e10 B)
+    sys.path.insert(0, ROOT)
+    import bench
+    t = bench.measured_traffic()
+    assert t is not None and t > 1.85e10
+    assert os.path.exists(bench.TRAFFIC_PROFILE)
