@@ -150,6 +150,13 @@ void tb_stage_destroy(tb_stage *st);
  * are OpenMP barriers). */
 int tb_stage_abort(tb_stage *st);
 
+/* The multi-resolution schedule reconstruct() runs (XT_Engine.c:111-141,
+ * one node): writes (mult_xy, mult_z) of each stage, coarse to fine, into
+ * arrays of at least 8 entries and returns the number of stages (-1 on
+ * sizes that are not positive).  Needs no GPU. */
+int32_t tb_multires_schedule(int32_t proj_rows, int32_t proj_cols, int32_t *mult_xy,
+                             int32_t *mult_z);
+
 /* ints: N_p N_r N_t N_x N_y N_z N_time mult_xy mult_z num_z_blocks
  * updateProjOffset updateVar initICD U ; dbls as the oracle's
  * orc_problem_info (delta_r .. delta_recon, then 27 filter weights). */

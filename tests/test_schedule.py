@@ -1,6 +1,8 @@
 """The multires schedule of reconstruct() (src/MBIR_4D/XT_Engine.c:111-141,
 one node): number of stages from the column count, mult_xy doubling while it
-divides the columns, mult_z doubling while rows / (2 mult_z) >= 2 (CPU only).
+divides the columns, mult_z doubling while rows / (2 mult_z) >= 2.  The
+schedule comes from the library's tb_multires_schedule, the code
+reconstruct() runs (no GPU needed).
 
 Known answers are the schedules the reference yields for the workloads'
 shapes; the C2 one is also what reconstruct() runs in the GPU tests."""

@@ -25,7 +25,7 @@ EXPORTS = [
     "tb_cost", "tb_offset_update",
     "tb_variance_update", "tb_voxel_update", "tb_update_colour", "tb_vsc_select",
     "tb_nhicd_list", "tb_icd_iteration", "tb_stage_run", "tb_get_stats", "tb_reset_stats",
-    "tb_synchronize", "tb_stage_object_device", "tb_stage_abort",
+    "tb_synchronize", "tb_stage_object_device", "tb_stage_abort", "tb_multires_schedule",
     # many problems in lockstep
     "tb_batch_create", "tb_batch_destroy", "tb_batch_iteration", "tb_batch_get_stats",
     # autotuner
@@ -128,6 +128,9 @@ def lib() -> C.CDLL:
     L.tb_reset_stats.argtypes = [vp]
     L.tb_synchronize.argtypes = [vp]
     L.tb_stage_abort.argtypes = [vp]
+    L.tb_multires_schedule.restype = C.c_int32
+    L.tb_multires_schedule.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32)]
     L.tb_stage_object_device.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_uint64)]
     L.tb_batch_create.argtypes = [C.POINTER(vp), i32]
     L.tb_batch_destroy.argtypes = [vp]

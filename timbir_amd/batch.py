@@ -23,27 +23,23 @@ out when it fires; results equal reconstruct_stages(), bit for bit.
 """
 from __future__ import annotations
 
-import math
 from concurrent.futures import ThreadPoolExecutor, as_completed
 
 import numpy as np
 
-MIN_XY_RECON_RES = 64.0  # XT_Constants.h:99
-
-
 def multires_schedule(proj_rows: int, proj_cols: int) -> list:
     """(mult_xy, mult_z) per stage, coarse to fine, as reconstruct() picks
-    them (XT_Engine.c:111-141 with one node)."""
-    nres = int(math.log(proj_cols / MIN_XY_RECON_RES) / math.log(2.0) + 1)
-    nres = max(2, min(8, nres))
-    mxy, mz = [1], [1]
-    for i in range(1, nres):
-        if proj_cols % (mxy[-1] * 2) != 0:
-            break
-        mxy.append(mxy[-1] * 2)
-        mz.append(mz[-1] * 2 if proj_rows % (mz[-1] * 2) == 0 and proj_rows // (mz[-1] * 2) >= 2
-                  else mz[-1])
-    return list(zip(mxy[::-1], mz[::-1]))
+    them (XT_Engine.c:111-141 with one node): the library's own
+    tb_multires_schedule, the code reconstruct() runs."""
+    import ctypes as C
+    from . import lib
+    L = lib()
+    mxy = (C.c_int32 * 8)()
+    mz = (C.c_int32 * 8)()
+    n = L.tb_multires_schedule(int(proj_rows), int(proj_cols), mxy, mz)
+    if n < 2:
+        raise ValueError(f"no multires schedule for {proj_rows} rows x {proj_cols} columns")
+    return [(int(mxy[i]), int(mz[i])) for i in range(n)]
 
 
 def reconstruct_stages(scan: dict, timing: list | None = None) -> np.ndarray:

@@ -78,6 +78,40 @@ int write_stage_outputs(tb_stage *st, int stage, const std::vector<float> &obj,
 
 }  // namespace
 
+using tb::kMaxMultres;
+using tb::kMinRowsPerNode;
+using tb::kMinXYRes;
+
+// The multi-resolution schedule of reconstruct() with one node
+// (XT_Engine.c:111-141): stages coarse to fine.
+extern "C" int32_t tb_multires_schedule(int32_t proj_rows, int32_t proj_cols, int32_t *mult_xy,
+                                        int32_t *mult_z) {
+  if (proj_cols <= 0 || proj_rows <= 0) return -1;
+  int nres = (int)(log(((float)proj_cols) / kMinXYRes) / log(2.0) + 1);
+  if (nres > kMaxMultres) nres = kMaxMultres;
+  if (nres < 2) nres = 2;
+  int mxy[kMaxMultres], mz[kMaxMultres];
+  mxy[0] = 1;
+  mz[0] = 1;
+  for (int i = 1; i < nres; i++) {
+    if (proj_cols % (mxy[i - 1] * 2) == 0) {
+      mxy[i] = mxy[i - 1] * 2;
+      if (proj_rows % (mz[i - 1] * 2) == 0 && proj_rows / (mz[i - 1] * 2) >= kMinRowsPerNode)
+        mz[i] = mz[i - 1] * 2;
+      else
+        mz[i] = mz[i - 1];
+    } else {
+      nres = i;
+      break;
+    }
+  }
+  for (int i = 0; i < nres; i++) {
+    if (mult_xy) mult_xy[i] = mxy[nres - 1 - i];
+    if (mult_z) mult_z[i] = mz[nres - 1 - i];
+  }
+  return nres;
+}
+
 extern "C" int reconstruct(float **object, float *projections, float *weights,
                            float *proj_angles, float *proj_times, float *recon_times,
                            int32_t proj_rows, int32_t proj_cols, int32_t proj_num,
@@ -109,29 +143,8 @@ extern "C" int reconstruct(float **object, float *projections, float *weights,
     return -1;
   }
   // multi-resolution schedule, XT_Engine.c:111-141
-  int nres = (int)(log(((float)proj_cols) / kMinXYRes) / log(2.0) + 1);
-  if (nres > kMaxMultres) nres = kMaxMultres;
-  if (nres < 2) nres = 2;
-  int mxy[kMaxMultres], mz[kMaxMultres];
-  mxy[0] = 1;
-  mz[0] = 1;
-  for (int i = 1; i < nres; i++) {
-    if (proj_cols % (mxy[i - 1] * 2) == 0) {
-      mxy[i] = mxy[i - 1] * 2;
-      if (proj_rows % (mz[i - 1] * 2) == 0 && proj_rows / (mz[i - 1] * 2) >= kMinRowsPerNode)
-        mz[i] = mz[i - 1] * 2;
-      else
-        mz[i] = mz[i - 1];
-    } else {
-      nres = i;
-      break;
-    }
-  }
-  int sxy[kMaxMultres], sz[kMaxMultres];
-  for (int i = 0; i < nres; i++) {
-    sxy[i] = mxy[nres - 1 - i];
-    sz[i] = mz[nres - 1 - i];
-  }
+  int32_t sxy[kMaxMultres], sz[kMaxMultres];
+  const int nres = tb_multires_schedule(proj_rows, proj_cols, sxy, sz);
   log_info(dbg, verbose, "Number of multi-resolution stages is %d.\n", nres);
 
   // restart (XT_Engine.c:143-157): the loop starts again at the stage
